@@ -1,0 +1,220 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, where it is the checker.  The product
+library never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MAX_COMP = 16
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("degree", ctypes.c_int),
+        ("n_cells", ctypes.c_int * 3),
+        ("h", ctypes.c_double * 3),
+        ("n_comp", ctypes.c_int),
+        ("tensorial", ctypes.c_int),
+        ("A", ctypes.c_double),
+        ("B", ctypes.c_double),
+        ("kappa_c", ctypes.c_double),
+        ("kappa_p", ctypes.c_double),
+        ("Mvol", ctypes.c_double),
+        ("Mvap", ctypes.c_double),
+        ("Msurf", ctypes.c_double),
+        ("Mgb", ctypes.c_double),
+        ("L", ctypes.c_double),
+        ("w0", ctypes.c_double),
+        ("w_alg", ctypes.c_double),
+        ("order", ctypes.c_int),
+        ("w_hist", ctypes.c_double * 3),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER(OrcParams)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_free_energy_eval.argtypes = [ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, dp, ctypes.c_int, dp]
+        L.orc_d2f_detaidetaj.argtypes = [ctypes.c_double] * 3
+        L.orc_d2f_detaidetaj.restype = ctypes.c_double
+        L.orc_power_sum.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        L.orc_power_sum.restype = ctypes.c_double
+        L.orc_unit_vector.argtypes = [dp, ctypes.c_int, ctypes.c_double, dp]
+        L.orc_unit_vector_filter_2.argtypes = [dp, ctypes.c_int, ctypes.c_double]
+        L.orc_unit_vector_filter_2.restype = ctypes.c_double
+        L.orc_residual_qp.argtypes = [P, ctypes.c_int, dp, dp, dp, dp, dp]
+        L.orc_tangent_qp.argtypes = [P, dp, dp, dp, dp, dp, dp]
+        L.orc_n_nodes.argtypes = [P]
+        L.orc_n_nodes.restype = ctypes.c_int64
+        L.orc_n_cells.argtypes = [P]
+        L.orc_n_cells.restype = ctypes.c_int64
+        L.orc_n_q_points.argtypes = [P]
+        L.orc_residual.argtypes = [P, ctypes.c_int, dp, ctypes.POINTER(dp), dp]
+        L.orc_fill_qp.argtypes = [P, dp, dp]
+        L.orc_vmult.argtypes = [P, dp, dp, dp]
+        L.orc_node_blocks.argtypes = [P, dp, dp]
+        L.orc_dense_tangent.argtypes = [P, dp, dp]
+        L.orc_max_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_params(dim, degree, n_cells, h, n_comp, tensorial=False, A=16.0,
+                B=1.0, kappa_c=1.0, kappa_p=0.5, Mvol=1e-2, Mvap=1e-10,
+                Msurf=4.0, Mgb=0.4, L=1.0, w0=0.0, w_alg=1.0, w_hist=()):
+    p = OrcParams()
+    p.dim = dim
+    p.degree = degree
+    for d in range(3):
+        p.n_cells[d] = int(n_cells[d]) if d < dim else 1
+        p.h[d] = float(h[d]) if d < dim else 1.0
+    p.n_comp = n_comp
+    p.tensorial = int(bool(tensorial))
+    p.A, p.B, p.kappa_c, p.kappa_p = A, B, kappa_c, kappa_p
+    p.Mvol, p.Mvap, p.Msurf, p.Mgb, p.L = Mvol, Mvap, Msurf, Mgb, L
+    p.w0 = w0
+    p.w_alg = w_alg
+    p.order = len(w_hist)
+    for i, w in enumerate(w_hist):
+        p.w_hist[i] = w
+    return p
+
+
+def n_nodes(p):
+    return int(lib().orc_n_nodes(ctypes.byref(p)))
+
+
+def n_cells(p):
+    return int(lib().orc_n_cells(ctypes.byref(p)))
+
+
+def n_q_points(p):
+    return int(lib().orc_n_q_points(ctypes.byref(p)))
+
+
+def free_energy_eval(A, B, c, etas):
+    etas = np.ascontiguousarray(etas, dtype=np.float64)
+    G = len(etas)
+    out = np.zeros(3 + 3 * G)
+    lib().orc_free_energy_eval(A, B, c, _dp(etas), G, _dp(out))
+    return {
+        "f": out[0], "df_dc": out[1], "d2f_dc2": out[2],
+        "df_detai": out[3:3 + G].copy(),
+        "d2f_dcdetai": out[3 + G:3 + 2 * G].copy(),
+        "d2f_detai2": out[3 + 2 * G:3 + 3 * G].copy(),
+    }
+
+
+def d2f_detaidetaj(B, ei, ej):
+    return lib().orc_d2f_detaidetaj(B, ei, ej)
+
+
+def power_sum(v, p):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_power_sum(_dp(v), len(v), p)
+
+
+def unit_vector(v, tol=1e-10):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros_like(v)
+    lib().orc_unit_vector(_dp(v), len(v), tol, _dp(out))
+    return out
+
+
+def unit_vector_filter_2(v, tol=1e-20):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_unit_vector_filter_2(_dp(v), len(v), tol)
+
+
+def residual_qp(p, wtd, value, grad, hist=None):
+    nc, dim = p.n_comp, p.dim
+    value = np.ascontiguousarray(value, dtype=np.float64)
+    grad = np.ascontiguousarray(grad, dtype=np.float64).reshape(nc * dim)
+    hist = np.zeros(nc) if hist is None else np.ascontiguousarray(hist, dtype=np.float64)
+    vr = np.zeros(nc)
+    gr = np.zeros(nc * dim)
+    lib().orc_residual_qp(ctypes.byref(p), wtd, _dp(value), _dp(grad), _dp(hist),
+                          _dp(vr), _dp(gr))
+    return vr, gr.reshape(nc, dim)
+
+
+def tangent_qp(p, lin_value, lin_grad, value, grad):
+    nc, dim = p.n_comp, p.dim
+    lv = np.ascontiguousarray(lin_value, dtype=np.float64)
+    lg = np.ascontiguousarray(lin_grad, dtype=np.float64).reshape(nc * dim)
+    v = np.ascontiguousarray(value, dtype=np.float64)
+    g = np.ascontiguousarray(grad, dtype=np.float64).reshape(nc * dim)
+    vr = np.zeros(nc)
+    gr = np.zeros(nc * dim)
+    lib().orc_tangent_qp(ctypes.byref(p), _dp(lv), _dp(lg), _dp(v), _dp(g),
+                         _dp(vr), _dp(gr))
+    return vr, gr.reshape(nc, dim)
+
+
+def residual(p, wtd, src, old=()):
+    src = np.ascontiguousarray(src, dtype=np.float64).ravel()
+    dst = np.empty_like(src)
+    olds = [np.ascontiguousarray(o, dtype=np.float64).ravel() for o in old]
+    arr = (ctypes.POINTER(ctypes.c_double) * max(1, len(olds)))()
+    for i, o in enumerate(olds):
+        arr[i] = _dp(o)
+    lib().orc_residual(ctypes.byref(p), wtd, _dp(src), arr, _dp(dst))
+    return dst
+
+
+def fill_qp(p, lin):
+    lin = np.ascontiguousarray(lin, dtype=np.float64).ravel()
+    size = n_cells(p) * n_q_points(p) * p.n_comp * (1 + p.dim)
+    cache = np.empty(size)
+    lib().orc_fill_qp(ctypes.byref(p), _dp(lin), _dp(cache))
+    return cache
+
+
+def vmult(p, cache, src):
+    src = np.ascontiguousarray(src, dtype=np.float64).ravel()
+    dst = np.empty_like(src)
+    lib().orc_vmult(ctypes.byref(p), _dp(cache), _dp(src), _dp(dst))
+    return dst
+
+
+def node_blocks(p, cache):
+    nn = n_nodes(p)
+    out = np.empty(nn * p.n_comp * p.n_comp)
+    lib().orc_node_blocks(ctypes.byref(p), _dp(cache), _dp(out))
+    return out.reshape(nn, p.n_comp, p.n_comp)
+
+
+def dense_tangent(p, cache):
+    n = n_nodes(p) * p.n_comp
+    out = np.empty(n * n)
+    lib().orc_dense_tangent(ctypes.byref(p), _dp(cache), _dp(out))
+    return out.reshape(n, n)
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
