@@ -1,0 +1,130 @@
+"""ctypes binding of libhpsint_amd.so (the C ABI in include/hpsint_amd.h).
+
+The library is the product: there is no fallback.  Loading fails loudly
+when it has not been built (``python -m hpsint_amd.build``).
+"""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhpsint_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "hpsint_amd.h")
+
+HPS_OK = 0
+HPS_ERR_INVALID_ARGUMENT = 1
+HPS_ERR_INVALID_N_COMP = 2
+HPS_ERR_UNSUPPORTED = 3
+HPS_ERR_OUT_OF_MEMORY = 4
+HPS_ERR_HIP = 5
+HPS_ERR_NOT_INITIALIZED = 6
+
+HPS_MOBILITY_SCALAR = 0
+HPS_MOBILITY_TENSORIAL = 1
+HPS_PRECON_POINT_JACOBI = 0
+HPS_PRECON_NODE_BLOCK_JACOBI = 1
+
+
+class HpsParams(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("fe_degree", ctypes.c_int),
+        ("n_cells", ctypes.c_int * 3),
+        ("cell_size", ctypes.c_double * 3),
+        ("n_components", ctypes.c_int),
+        ("mobility", ctypes.c_int),
+        ("A", ctypes.c_double),
+        ("B", ctypes.c_double),
+        ("kappa_c", ctypes.c_double),
+        ("kappa_p", ctypes.c_double),
+        ("Mvol", ctypes.c_double),
+        ("Mvap", ctypes.c_double),
+        ("Msurf", ctypes.c_double),
+        ("Mgb", ctypes.c_double),
+        ("L", ctypes.c_double),
+    ]
+
+
+class HpsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hpsint_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class InvalidNumberOfComponents(HpsError):
+    """ExcInvalidNumberOfComponents (S/instantiation.h:61-68)"""
+
+
+_LIB = None
+
+vp = ctypes.c_void_p
+i64 = ctypes.c_int64
+dbl = ctypes.c_double
+cint = ctypes.c_int
+pd = ctypes.POINTER(ctypes.c_double)
+
+_SIGNATURES = {
+    "hps_create": [ctypes.POINTER(HpsParams), ctypes.POINTER(vp)],
+    "hps_destroy": [vp],
+    "hps_set_stream": [vp, vp],
+    "hps_sync": [vp],
+    "hps_sizes": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
+    "hps_set_time_weights": [vp, dbl, dbl, pd, cint],
+    "hps_set_linearization": [vp, vp],
+    "hps_residual": [vp, cint, vp, vp, ctypes.POINTER(vp)],
+    "hps_vmult_f64": [vp, vp, vp],
+    "hps_vmult_f32": [vp, vp, vp],
+    "hps_precond_update": [vp, cint],
+    "hps_precond_apply_f64": [vp, vp, vp],
+    "hps_precond_apply_f32": [vp, vp, vp],
+    "hps_precond_get_diagonal": [vp, vp],
+    "hps_precond_get_blocks": [vp, vp],
+    "hps_dot": [vp, i64, vp, vp, pd],
+    "hps_norm2": [vp, i64, vp, pd],
+    "hps_block_norms": [vp, vp, pd],
+    "hps_axpy": [vp, i64, dbl, vp, vp],
+    "hps_sadd": [vp, i64, dbl, dbl, vp, vp],
+    "hps_scale": [vp, i64, dbl, vp],
+    "hps_copy": [vp, i64, vp, vp],
+    "hps_fill": [vp, i64, dbl, vp],
+    "hps_multi_dot": [vp, i64, cint, vp, vp, pd],
+    "hps_multi_axpy_norm": [vp, i64, cint, vp, pd, vp, pd],
+    "hps_convert_f64_to_f32": [vp, i64, vp, vp],
+    "hps_convert_f32_to_f64": [vp, i64, vp, vp],
+    "hps_from_blocks": [vp, vp, ctypes.POINTER(vp)],
+    "hps_to_blocks": [vp, ctypes.POINTER(vp), vp],
+    "hps_last_kernel_ms": [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "hps_enable_timing": [vp, cint],
+}
+
+
+def header_symbols():
+    """Names of the entry points declared in include/hpsint_amd.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(hps_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libhpsint_amd.so is not built (run `python -m hpsint_amd.build`); "
+                "there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = cint
+        L.hps_last_error.restype = ctypes.c_char_p
+        L.hps_last_error.argtypes = []
+        _LIB = L
+    return _LIB
+
+
+def check(code):
+    if code != HPS_OK:
+        msg = lib().hps_last_error().decode()
+        if code == HPS_ERR_INVALID_N_COMP:
+            raise InvalidNumberOfComponents(code, msg)
+        raise HpsError(code, msg)

@@ -1,0 +1,683 @@
+// api.hip -- C ABI of the MI355X sintering-operator library (see
+// include/hpsint_amd.h for the contract of every entry point).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/hpsint_amd.h"
+#include "blas.hpp"
+#include "ops.hpp"
+
+namespace
+{
+  thread_local std::string g_last_error;
+
+  int fail(int code, const std::string &msg)
+  {
+    g_last_error = msg;
+    return code;
+  }
+
+#define HPS_HIP(call)                                                              \
+  do                                                                               \
+    {                                                                              \
+      const hipError_t e_ = (call);                                                \
+      if (e_ != hipSuccess)                                                        \
+        return fail(e_ == hipErrorOutOfMemory ? HPS_ERR_OUT_OF_MEMORY : HPS_ERR_HIP, \
+                    std::string(#call) + ": " + hipGetErrorString(e_));            \
+    }                                                                              \
+  while (0)
+
+#define HPS_LAUNCHED()                                                             \
+  do                                                                               \
+    {                                                                              \
+      const hipError_t e_ = hipGetLastError();                                     \
+      if (e_ != hipSuccess)                                                        \
+        return fail(HPS_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    }                                                                              \
+  while (0)
+
+  constexpr int TZ = 16; // cells per tile along the march direction
+} // namespace
+
+struct hps_operator_s
+{
+  hps_params           p;
+  int                  dim, n_grains, nc;
+  bool                 tens;
+  const hps::OpTable  *ops;
+  hps::KParams         kp;
+  hps::FixParams       fp;
+  int                  n_tiles;
+  int64_t              n_nodes, n_dofs, cache_elems, part_elems;
+  hipStream_t          stream;
+  double              *cache64;
+  float               *cache32;
+  bool                 lin_set;
+  double              *part;
+  double              *red_partial, *red_out, *red_host;
+  double             **ptrs_dev;
+  double             **ptrs_host;
+  int                  precond_kind;
+  double              *diag, *inv_diag64, *blocks, *inv_blocks64;
+  float               *inv_diag32, *inv_blocks32;
+  int                 *d_singular;
+  hipEvent_t           ev[3];
+  bool                 timing;
+  const double        *old[3];
+};
+
+static int check_op(hps_operator op)
+{
+  if (!op)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "null operator handle");
+  return HPS_OK;
+}
+
+extern "C" const char *hps_last_error(void)
+{
+  return g_last_error.c_str();
+}
+
+extern "C" int hps_create(const hps_params *prm, hps_operator *out)
+{
+  if (!prm || !out)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (prm->dim != 2 && prm->dim != 3)
+    return fail(HPS_ERR_UNSUPPORTED, "dim must be 2 or 3");
+  if (prm->fe_degree != 1)
+    return fail(HPS_ERR_UNSUPPORTED, "only FE_Q(1) with QGauss(2) is built");
+  if (prm->mobility != HPS_MOBILITY_SCALAR && prm->mobility != HPS_MOBILITY_TENSORIAL)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "unknown mobility");
+  for (int d = 0; d < prm->dim; ++d)
+    if (prm->n_cells[d] < 1 || !(prm->cell_size[d] > 0.0))
+      return fail(HPS_ERR_INVALID_ARGUMENT, "cells and cell sizes must be positive");
+  const int  G    = prm->n_components - 2;
+  const bool tens = prm->mobility == HPS_MOBILITY_TENSORIAL;
+  if (G < 1)
+    return fail(HPS_ERR_INVALID_N_COMP, "n_components must be >= 3");
+  const hps::OpTable *ops = prm->dim == 3 ? hps::q1_table_d3(G, tens) : hps::q1_table_d2(G, tens);
+  if (!ops)
+    return fail(HPS_ERR_INVALID_N_COMP,
+                "n_components = " + std::to_string(prm->n_components) +
+                  " is not instantiated for this dim/mobility");
+
+  hps_operator op = new hps_operator_s();
+  std::memset((void *)op, 0, sizeof(*op));
+  op->p            = *prm;
+  op->dim          = prm->dim;
+  op->n_grains     = G;
+  op->nc           = G + 2;
+  op->tens         = tens;
+  op->ops          = ops;
+  op->precond_kind = -1;
+
+  hps::KParams &kp = op->kp;
+  const int     ncx = prm->n_cells[0];
+  const int     ncy = prm->dim == 3 ? prm->n_cells[1] : 1;
+  const int     ncm = prm->dim == 3 ? prm->n_cells[2] : prm->n_cells[1];
+  kp.nc[0]         = ncx;
+  kp.nc[1]         = ncy;
+  kp.nc[2]         = ncm;
+  kp.nn[0]         = ncx + 1;
+  kp.nn[1]         = prm->dim == 3 ? ncy + 1 : 1;
+  kp.nn[2]         = ncm + 1;
+  kp.nt[0]         = (ncx + ops->tx - 1) / ops->tx;
+  kp.nt[1]         = (ncy + ops->ty - 1) / ops->ty;
+  kp.nt[2]         = (ncm + TZ - 1) / TZ;
+  kp.tz            = TZ;
+  double vol       = 1.0;
+  for (int d = 0; d < 3; ++d)
+    kp.inv_h[d] = d < prm->dim ? 1.0 / prm->cell_size[d] : 1.0;
+  for (int d = 0; d < prm->dim; ++d)
+    vol *= prm->cell_size[d] * 0.5;
+  kp.jxw        = vol;
+  kp.pp.A       = prm->A;
+  kp.pp.B       = prm->B;
+  kp.pp.kappa_c = prm->kappa_c;
+  kp.pp.kappa_p = prm->kappa_p;
+  kp.pp.Mvol    = prm->Mvol;
+  kp.pp.Mvap    = prm->Mvap;
+  kp.pp.Msurf   = prm->Msurf;
+  kp.pp.Mgb     = prm->Mgb;
+  kp.pp.L       = prm->L;
+  kp.pp.w0      = 0.0;
+  kp.pp.w_alg   = 1.0;
+  kp.order      = 0;
+
+  op->n_tiles = kp.nt[0] * kp.nt[1] * kp.nt[2];
+  op->n_nodes = (int64_t)kp.nn[0] * kp.nn[1] * kp.nn[2];
+  op->n_dofs  = op->n_nodes * op->nc;
+
+  const int px = ops->tx + 1;
+  const int pn = px * (prm->dim == 3 ? ops->ty + 1 : 1);
+  op->cache_elems = (int64_t)op->n_tiles * TZ * ops->n_fields * ops->nt;
+  op->part_elems  = (int64_t)op->n_tiles * (TZ + 1) * pn * op->nc;
+
+  hps::FixParams &fp = op->fp;
+  for (int d = 0; d < 3; ++d)
+    {
+      fp.nc[d] = kp.nc[d];
+      fp.nn[d] = kp.nn[d];
+      fp.nt[d] = kp.nt[d];
+      fp.nb[d] = kp.nt[d] - 1;
+    }
+  fp.tsz[0] = ops->tx;
+  fp.tsz[1] = prm->dim == 3 ? ops->ty : 1;
+  fp.tsz[2] = TZ;
+  fp.n_comp = op->nc;
+  fp.px     = px;
+  fp.pn     = pn;
+  fp.n_m    = (int64_t)fp.nb[2] * fp.nn[0] * fp.nn[1];
+  fp.n_y    = (int64_t)fp.nb[1] * fp.nn[0] * fp.nn[2];
+  fp.n_x    = (int64_t)fp.nb[0] * fp.nn[1] * fp.nn[2];
+
+  auto cleanup_fail = [&](hipError_t e) {
+    hps_destroy(op);
+    return fail(e == hipErrorOutOfMemory ? HPS_ERR_OUT_OF_MEMORY : HPS_ERR_HIP,
+                std::string("allocation: ") + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&op->part, op->part_elems * sizeof(double))) != hipSuccess)
+    return cleanup_fail(e);
+  const int64_t red = (int64_t)hps::RED_BLOCKS * (hps::MAXK > op->nc ? hps::MAXK : op->nc);
+  if ((e = hipMalloc(&op->red_partial, red * sizeof(double))) != hipSuccess)
+    return cleanup_fail(e);
+  if ((e = hipMalloc(&op->red_out, 64 * sizeof(double))) != hipSuccess)
+    return cleanup_fail(e);
+  if ((e = hipHostMalloc(&op->red_host, 64 * sizeof(double))) != hipSuccess)
+    return cleanup_fail(e);
+  if ((e = hipMalloc(&op->ptrs_dev, 32 * sizeof(double *))) != hipSuccess)
+    return cleanup_fail(e);
+  if ((e = hipHostMalloc(&op->ptrs_host, 32 * sizeof(double *))) != hipSuccess)
+    return cleanup_fail(e);
+  if ((e = hipMalloc(&op->d_singular, sizeof(int))) != hipSuccess)
+    return cleanup_fail(e);
+  for (int i = 0; i < 3; ++i)
+    if ((e = hipEventCreate(&op->ev[i])) != hipSuccess)
+      return cleanup_fail(e);
+  *out = op;
+  return HPS_OK;
+}
+
+extern "C" int hps_destroy(hps_operator op)
+{
+  if (!op)
+    return HPS_OK;
+  (void)hipStreamSynchronize(op->stream);
+  hipFree(op->cache64);
+  hipFree(op->cache32);
+  hipFree(op->part);
+  hipFree(op->red_partial);
+  hipFree(op->red_out);
+  if (op->red_host)
+    hipHostFree(op->red_host);
+  hipFree(op->ptrs_dev);
+  if (op->ptrs_host)
+    hipHostFree(op->ptrs_host);
+  hipFree(op->diag);
+  hipFree(op->inv_diag64);
+  hipFree(op->inv_diag32);
+  hipFree(op->blocks);
+  hipFree(op->inv_blocks64);
+  hipFree(op->inv_blocks32);
+  hipFree(op->d_singular);
+  for (int i = 0; i < 3; ++i)
+    if (op->ev[i])
+      hipEventDestroy(op->ev[i]);
+  delete op;
+  return HPS_OK;
+}
+
+extern "C" int hps_set_stream(hps_operator op, void *stream)
+{
+  if (int r = check_op(op))
+    return r;
+  op->stream = (hipStream_t)stream;
+  return HPS_OK;
+}
+
+extern "C" int hps_sync(hps_operator op)
+{
+  if (int r = check_op(op))
+    return r;
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_sizes(hps_operator op, int64_t *n_nodes, int64_t *n_dofs)
+{
+  if (int r = check_op(op))
+    return r;
+  if (n_nodes)
+    *n_nodes = op->n_nodes;
+  if (n_dofs)
+    *n_dofs = op->n_dofs;
+  return HPS_OK;
+}
+
+extern "C" int hps_set_time_weights(hps_operator op, double w0, double w_alg,
+                                    const double *w_hist, int order)
+{
+  if (int r = check_op(op))
+    return r;
+  if (order < 0 || order > 3 || (order > 0 && !w_hist))
+    return fail(HPS_ERR_INVALID_ARGUMENT, "order must be 0..3 with weights");
+  op->kp.pp.w0    = w0;
+  op->kp.pp.w_alg = w_alg;
+  op->kp.order    = order;
+  for (int i = 0; i < 3; ++i)
+    op->kp.w_hist[i] = i < order ? w_hist[i] : 0.0;
+  return HPS_OK;
+}
+
+extern "C" int hps_set_linearization(hps_operator op, const double *u)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!u)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "null vector");
+  if (!op->cache64)
+    {
+      HPS_HIP(hipMalloc(&op->cache64, op->cache_elems * sizeof(double)));
+      HPS_HIP(hipMalloc(&op->cache32, op->cache_elems * sizeof(float)));
+    }
+  op->ops->fill(op->kp, op->n_tiles, u, op->cache64, op->cache32, op->stream);
+  HPS_LAUNCHED();
+  op->lin_set = true;
+  return HPS_OK;
+}
+
+template <typename T>
+static int finish_fixup(hps_operator op, T *dst, int ostride, int ocstride, int ooffset)
+{
+  hps::FixParams fp = op->fp;
+  fp.ostride        = ostride;
+  fp.ocstride       = ocstride;
+  fp.ooffset        = ooffset;
+  hps::launch_fixup<T>(fp, (const T *)op->part, dst, op->stream);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double *src,
+                            const double *const *old)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!dst || !src || wtd < 0 || wtd > 2)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "residual: bad arguments");
+  const double *o[3] = {nullptr, nullptr, nullptr};
+  if (wtd == 2)
+    {
+      if (op->kp.order > 0 && !old)
+        return fail(HPS_ERR_INVALID_ARGUMENT, "residual: history vectors missing");
+      for (int i = 0; i < op->kp.order; ++i)
+        {
+          if (!old[i])
+            return fail(HPS_ERR_INVALID_ARGUMENT, "residual: null history vector");
+          o[i] = old[i];
+        }
+    }
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[0], op->stream));
+  // with order 0 there is no history: evaluate without the history term
+  const int w = (wtd == 2 && op->kp.order == 0) ? 1 : wtd;
+  op->ops->residual(op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part, op->stream);
+  HPS_LAUNCHED();
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[1], op->stream));
+  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0))
+    return r;
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!dst || !src)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "vmult: null vector");
+  if (!op->lin_set)
+    return fail(HPS_ERR_NOT_INITIALIZED, "vmult before hps_set_linearization");
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[0], op->stream));
+  op->ops->vmult64(op->kp, op->n_tiles, src, dst, op->part, op->cache64, op->stream);
+  HPS_LAUNCHED();
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[1], op->stream));
+  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0))
+    return r;
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!dst || !src)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "vmult: null vector");
+  if (!op->lin_set)
+    return fail(HPS_ERR_NOT_INITIALIZED, "vmult before hps_set_linearization");
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[0], op->stream));
+  op->ops->vmult32(op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32, op->stream);
+  HPS_LAUNCHED();
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[1], op->stream));
+  if (int r = finish_fixup<float>(op, dst, op->nc, 1, 0))
+    return r;
+  if (op->timing)
+    HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_precond_update(hps_operator op, int kind)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!op->lin_set)
+    return fail(HPS_ERR_NOT_INITIALIZED, "preconditioner update before hps_set_linearization");
+  const int64_t n = op->n_dofs, nn = op->n_nodes;
+  const int     nc = op->nc;
+  if (kind == HPS_PRECON_POINT_JACOBI)
+    {
+      if (!op->diag)
+        {
+          HPS_HIP(hipMalloc(&op->diag, n * sizeof(double)));
+          HPS_HIP(hipMalloc(&op->inv_diag64, n * sizeof(double)));
+          HPS_HIP(hipMalloc(&op->inv_diag32, n * sizeof(float)));
+        }
+      op->ops->probe(op->kp, op->n_tiles, op->cache64, op->diag, op->part, -1, nc, 1, 0,
+                     op->stream);
+      HPS_LAUNCHED();
+      if (int r = finish_fixup<double>(op, op->diag, nc, 1, 0))
+        return r;
+      hps::invert_diag(op->stream, n, op->diag, op->inv_diag64, op->inv_diag32);
+      HPS_LAUNCHED();
+    }
+  else if (kind == HPS_PRECON_NODE_BLOCK_JACOBI)
+    {
+      if (!op->blocks)
+        {
+          HPS_HIP(hipMalloc(&op->blocks, nn * nc * nc * sizeof(double)));
+          HPS_HIP(hipMalloc(&op->inv_blocks64, nn * nc * nc * sizeof(double)));
+          HPS_HIP(hipMalloc(&op->inv_blocks32, nn * nc * nc * sizeof(float)));
+        }
+      for (int j = 0; j < nc; ++j)
+        {
+          op->ops->probe(op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc, nc,
+                         j, op->stream);
+          HPS_LAUNCHED();
+          if (int r = finish_fixup<double>(op, op->blocks, nc * nc, nc, j))
+            return r;
+        }
+      HPS_HIP(hipMemsetAsync(op->d_singular, 0, sizeof(int), op->stream));
+      if (!hps::invert_blocks(op->stream, nn, nc, op->blocks, op->inv_blocks64,
+                              op->inv_blocks32, op->d_singular))
+        return fail(HPS_ERR_INVALID_N_COMP, "node-block Jacobi not built for this n_components");
+      HPS_LAUNCHED();
+      int singular = 0;
+      HPS_HIP(hipMemcpyAsync(&singular, op->d_singular, sizeof(int), hipMemcpyDeviceToHost,
+                             op->stream));
+      HPS_HIP(hipStreamSynchronize(op->stream));
+      if (singular)
+        return fail(HPS_ERR_INVALID_ARGUMENT,
+                    std::to_string(singular) + " singular node blocks");
+    }
+  else
+    return fail(HPS_ERR_INVALID_ARGUMENT, "unknown preconditioner kind");
+  op->precond_kind = kind;
+  return HPS_OK;
+}
+
+extern "C" int hps_precond_apply_f64(hps_operator op, double *dst, const double *src)
+{
+  if (int r = check_op(op))
+    return r;
+  if (op->precond_kind == HPS_PRECON_POINT_JACOBI)
+    hps::diag_apply64(op->stream, op->n_dofs, op->inv_diag64, src, dst);
+  else if (op->precond_kind == HPS_PRECON_NODE_BLOCK_JACOBI)
+    hps::block_apply64(op->stream, op->n_nodes, op->nc, op->inv_blocks64, src, dst);
+  else
+    return fail(HPS_ERR_NOT_INITIALIZED, "preconditioner not set up");
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_precond_apply_f32(hps_operator op, float *dst, const float *src)
+{
+  if (int r = check_op(op))
+    return r;
+  if (op->precond_kind == HPS_PRECON_POINT_JACOBI)
+    hps::diag_apply32(op->stream, op->n_dofs, op->inv_diag32, src, dst);
+  else if (op->precond_kind == HPS_PRECON_NODE_BLOCK_JACOBI)
+    hps::block_apply32(op->stream, op->n_nodes, op->nc, op->inv_blocks32, src, dst);
+  else
+    return fail(HPS_ERR_NOT_INITIALIZED, "preconditioner not set up");
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_precond_get_diagonal(hps_operator op, double *inv_diag)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!op->inv_diag64)
+    return fail(HPS_ERR_NOT_INITIALIZED, "no diagonal computed");
+  HPS_HIP(hipMemcpyAsync(inv_diag, op->inv_diag64, op->n_dofs * sizeof(double),
+                         hipMemcpyDeviceToDevice, op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_precond_get_blocks(hps_operator op, double *blocks)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!op->blocks)
+    return fail(HPS_ERR_NOT_INITIALIZED, "no node blocks computed");
+  HPS_HIP(hipMemcpyAsync(blocks, op->blocks, op->n_nodes * op->nc * op->nc * sizeof(double),
+                         hipMemcpyDeviceToDevice, op->stream));
+  return HPS_OK;
+}
+
+// ------------------------------------------------------------------ BLAS-1
+static int fetch(hps_operator op, int k, double *result)
+{
+  HPS_LAUNCHED();
+  HPS_HIP(hipMemcpyAsync(op->red_host, op->red_out, k * sizeof(double), hipMemcpyDeviceToHost,
+                         op->stream));
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  for (int i = 0; i < k; ++i)
+    result[i] = op->red_host[i];
+  return HPS_OK;
+}
+
+extern "C" int hps_dot(hps_operator op, int64_t n, const double *x, const double *y,
+                       double *result)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_dot(op->stream, n, x, y, op->red_partial, op->red_out);
+  return fetch(op, 1, result);
+}
+
+extern "C" int hps_norm2(hps_operator op, int64_t n, const double *x, double *result)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_norm2(op->stream, n, x, op->red_partial, op->red_out);
+  return fetch(op, 1, result);
+}
+
+extern "C" int hps_block_norms(hps_operator op, const double *x, double *result)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_block_norms(op->stream, op->n_nodes, op->nc, x, op->red_partial, op->red_out);
+  return fetch(op, op->nc, result);
+}
+
+extern "C" int hps_multi_dot(hps_operator op, int64_t n, int k, const double *V,
+                             const double *w, double *h)
+{
+  if (int r = check_op(op))
+    return r;
+  if (k < 1 || k > hps::MAXK)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "multi_dot: k must be 1..32");
+  hps::blas_multi_dot(op->stream, n, k, V, w, op->red_partial, op->red_out);
+  return fetch(op, k, h);
+}
+
+extern "C" int hps_multi_axpy_norm(hps_operator op, int64_t n, int k, const double *V,
+                                   const double *h, double *w, double *norm)
+{
+  if (int r = check_op(op))
+    return r;
+  if (k < 1 || k > hps::MAXK)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "multi_axpy_norm: k must be 1..32");
+  // coefficients go through the pinned buffer into device memory
+  for (int j = 0; j < k; ++j)
+    op->red_host[32 + j] = h[j];
+  HPS_HIP(hipMemcpyAsync(op->red_out + 32, op->red_host + 32, k * sizeof(double),
+                         hipMemcpyHostToDevice, op->stream));
+  hps::blas_multi_axpy_norm(op->stream, n, k, V, op->red_out + 32, w, op->red_partial,
+                            op->red_out);
+  return fetch(op, 1, norm);
+}
+
+extern "C" int hps_axpy(hps_operator op, int64_t n, double a, const double *x, double *y)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_axpy(op->stream, n, a, x, y);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_sadd(hps_operator op, int64_t n, double s, double a, const double *x,
+                        double *y)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_sadd(op->stream, n, s, a, x, y);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_scale(hps_operator op, int64_t n, double a, double *x)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_scale(op->stream, n, a, x);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_copy(hps_operator op, int64_t n, double *dst, const double *src)
+{
+  if (int r = check_op(op))
+    return r;
+  HPS_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_fill(hps_operator op, int64_t n, double value, double *x)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_fill(op->stream, n, value, x);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_convert_f64_to_f32(hps_operator op, int64_t n, float *dst, const double *src)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_f64_to_f32(op->stream, n, dst, src);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_convert_f32_to_f64(hps_operator op, int64_t n, double *dst, const float *src)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_f32_to_f64(op->stream, n, dst, src);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+static int push_ptrs(hps_operator op, const void *const *ptrs)
+{
+  for (int c = 0; c < op->nc; ++c)
+    {
+      if (!ptrs[c])
+        return fail(HPS_ERR_INVALID_ARGUMENT, "null block pointer");
+      op->ptrs_host[c] = (double *)ptrs[c];
+    }
+  HPS_HIP(hipMemcpyAsync(op->ptrs_dev, op->ptrs_host, op->nc * sizeof(double *),
+                         hipMemcpyHostToDevice, op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_from_blocks(hps_operator op, double *x, const double *const *blocks)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!x || !blocks)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "null argument");
+  if (int r = push_ptrs(op, (const void *const *)blocks))
+    return r;
+  hps::from_blocks(op->stream, op->n_nodes, op->nc, x, (const double *const *)op->ptrs_dev);
+  HPS_LAUNCHED();
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_to_blocks(hps_operator op, double *const *blocks, const double *x)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!x || !blocks)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "null argument");
+  if (int r = push_ptrs(op, (const void *const *)blocks))
+    return r;
+  hps::to_blocks(op->stream, op->n_nodes, op->nc, op->ptrs_dev, x);
+  HPS_LAUNCHED();
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  return HPS_OK;
+}
+
+extern "C" int hps_enable_timing(hps_operator op, int enable)
+{
+  if (int r = check_op(op))
+    return r;
+  op->timing = enable != 0;
+  return HPS_OK;
+}
+
+extern "C" int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fixup)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!op->timing)
+    return fail(HPS_ERR_NOT_INITIALIZED, "timing not enabled");
+  HPS_HIP(hipEventSynchronize(op->ev[2]));
+  float a = 0.f, b = 0.f;
+  HPS_HIP(hipEventElapsedTime(&a, op->ev[0], op->ev[1]));
+  HPS_HIP(hipEventElapsedTime(&b, op->ev[1], op->ev[2]));
+  if (ms_main)
+    *ms_main = a;
+  if (ms_fixup)
+    *ms_fixup = b;
+  return HPS_OK;
+}

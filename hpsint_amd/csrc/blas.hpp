@@ -1,0 +1,39 @@
+// blas.hpp -- launchers of the block-vector BLAS-1 and Jacobi kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hps
+{
+  constexpr int RED_BLOCKS = 1024; // fixed reduction grid (deterministic sums)
+  constexpr int MAXK       = 32;   // max vectors of one multi-dot
+
+  void blas_dot(hipStream_t, int64_t n, const double *x, const double *y, double *partial,
+                double *out);
+  void blas_norm2(hipStream_t, int64_t n, const double *x, double *partial, double *out);
+  void blas_block_norms(hipStream_t, int64_t n_nodes, int nc, const double *x,
+                        double *partial, double *out);
+  void blas_multi_dot(hipStream_t, int64_t n, int k, const double *V, const double *w,
+                      double *partial, double *out);
+  void blas_multi_axpy_norm(hipStream_t, int64_t n, int k, const double *V, const double *h,
+                            double *w, double *partial, double *out);
+  void blas_axpy(hipStream_t, int64_t n, double a, const double *x, double *y);
+  void blas_sadd(hipStream_t, int64_t n, double s, double a, const double *x, double *y);
+  void blas_scale(hipStream_t, int64_t n, double a, double *x);
+  void blas_fill(hipStream_t, int64_t n, double a, double *x);
+  void blas_f64_to_f32(hipStream_t, int64_t n, float *dst, const double *src);
+  void blas_f32_to_f64(hipStream_t, int64_t n, double *dst, const float *src);
+  void invert_diag(hipStream_t, int64_t n, const double *d, double *inv64, float *inv32);
+  void diag_apply64(hipStream_t, int64_t n, const double *inv, const double *src, double *dst);
+  void diag_apply32(hipStream_t, int64_t n, const float *inv, const float *src, float *dst);
+  bool invert_blocks(hipStream_t, int64_t n_nodes, int nc, const double *blocks, double *inv64,
+                     float *inv32, int *singular);
+  bool block_apply64(hipStream_t, int64_t n_nodes, int nc, const double *inv, const double *src,
+                     double *dst);
+  bool block_apply32(hipStream_t, int64_t n_nodes, int nc, const float *inv, const float *src,
+                     float *dst);
+  void from_blocks(hipStream_t, int64_t n_nodes, int nc, double *x, const double *const *blocks_dev);
+  void to_blocks(hipStream_t, int64_t n_nodes, int nc, double *const *blocks_dev, const double *x);
+} // namespace hps

@@ -1,0 +1,799 @@
+// fe_q1.hpp -- Q1 cell kernels of the sintering operator for gfx950.
+//
+// What the reference computes with deal.II FEEvaluation on the CPU
+// (gather, sum-factorised interpolation, quadrature-point physics,
+// integration, scatter-add; S/operator_sintering_generic.h:539-577 for the
+// tangent, :877-1006 for the residual, S/operator_sintering_data.h:301-409
+// for the linearisation cache) is done here by one kernel per operation.
+//
+// Lane mapping.  One lane per (cell, corner): the 2^DIM lanes of a cell hold
+// the cell's nodal values on entry, the quadrature-point values after
+// interpolation and the nodal contributions after integration.  Lane bit d
+// is the node/q-point index along direction d.  The 2-point Gauss
+// interpolation along d is a butterfly between lanes that differ in bit d
+// (DPP quad_perm for bits 0 and 1, ds_swizzle for bit 2):
+//   value: sa*own + sb*partner            derivative: sg*(own - partner)/h
+// and integration is its transpose:
+//   value: sa*own + sb*partner            derivative: sg*(own + partner)
+// with sa = (1+1/sqrt3)/2, sb = (1-1/sqrt3)/2, sg = +1 for bit 1, -1 for 0.
+//
+// Tiling.  A workgroup owns a column of TX x TY cells (x, y) and marches
+// through a chunk of `tz` cell layers along the last direction (z in 3D, y
+// in 2D).  Two planes of source nodes live in LDS; the planes one layer
+// ahead are fetched into registers while the current layer computes.  Every
+// lane writes its nodal contribution to an LDS slot indexed by (corner,
+// node), so no two lanes write the same slot; after a barrier each node sums
+// its slots in a fixed order (deterministic, no atomics).  The bottom corners
+// complete node plane k, the top corners start plane k+1 (kept in registers
+// as the carry).  Nodes shared with a neighbouring workgroup are written to
+// a per-tile partial buffer and summed by q1_fixup in a fixed order.
+//
+// The linearisation cache is stored in lane order, cache[tile][layer]
+// [field][lane], so each field of a layer is one coalesced 2 KiB read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "physics.hpp"
+
+namespace hps
+{
+  struct KParams
+  {
+    int    nc[3];     // cells along x, y (1 in 2D), march direction
+    int    nn[3];     // nodes along x, y (1 in 2D), march direction
+    int    nt[3];     // tiles along x, y, march
+    int    tz;        // cells per tile along the march direction
+    int    order;     // number of history vectors (BDF order)
+    double inv_h[3];  // 1/h along the physical directions x, y, z
+    double jxw;       // prod(h) * (1/2)^DIM
+    double w_hist[3]; // BDF weights[1..order]
+    PhysParams pp;
+  };
+
+  enum Op
+  {
+    OP_VMULT = 0,
+    OP_RES0  = 1, // residual, no time terms
+    OP_RES1  = 2, // residual, primary weight only
+    OP_RES2  = 3, // residual with BDF history
+  };
+
+  // ---------------------------------------------------------------- lanes
+  template <int MASK>
+  __device__ __forceinline__ int xor_lane(int v);
+  template <>
+  __device__ __forceinline__ int xor_lane<1>(int v)
+  {
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
+  }
+  template <>
+  __device__ __forceinline__ int xor_lane<2>(int v)
+  {
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false); // quad_perm [2,3,0,1]
+  }
+  template <>
+  __device__ __forceinline__ int xor_lane<4>(int v)
+  {
+    return __builtin_amdgcn_ds_swizzle(v, 0x101F); // bitmask mode, xor 4
+  }
+
+  template <int MASK>
+  __device__ __forceinline__ double xshfl(double v)
+  {
+    const int lo = __double2loint(v);
+    const int hi = __double2hiint(v);
+    return __hiloint2double(xor_lane<MASK>(hi), xor_lane<MASK>(lo));
+  }
+  template <int MASK>
+  __device__ __forceinline__ float xshfl(float v)
+  {
+    return __int_as_float(xor_lane<MASK>(__float_as_int(v)));
+  }
+
+  template <typename T>
+  struct Q1Coef
+  {
+    static constexpr T sa = T(0.78867513459481288225457439025098);
+    static constexpr T sb = T(0.21132486540518711774542560974902);
+  };
+
+  // values and gradients at the lane's q-point from the lane's nodal value
+  template <int DIM, typename T>
+  __device__ __forceinline__ void interp_vg(T u, int corner, const T *inv_h, T &val,
+                                            T *grad)
+  {
+    const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    const T s0 = (corner & 1) ? T(1) : T(-1);
+    const T s1 = (corner & 2) ? T(1) : T(-1);
+    const T p  = xshfl<1>(u);
+    const T A  = sa * u + sb * p;
+    const T Dx = s0 * (u - p);
+    const T pA = xshfl<2>(A);
+    const T pD = xshfl<2>(Dx);
+    if constexpr (DIM == 2)
+      {
+        val     = sa * A + sb * pA;
+        grad[1] = s1 * (A - pA) * inv_h[1];
+        grad[0] = (sa * Dx + sb * pD) * inv_h[0];
+      }
+    else
+      {
+        const T s2   = (corner & 4) ? T(1) : T(-1);
+        const T AA   = sa * A + sb * pA;
+        const T Ay   = s1 * (A - pA);
+        const T DxA  = sa * Dx + sb * pD;
+        const T pAA  = xshfl<4>(AA);
+        const T pAy  = xshfl<4>(Ay);
+        const T pDxA = xshfl<4>(DxA);
+        val          = sa * AA + sb * pAA;
+        grad[2]      = s2 * (AA - pAA) * inv_h[2];
+        grad[1]      = (sa * Ay + sb * pAy) * inv_h[1];
+        grad[0]      = (sa * DxA + sb * pDxA) * inv_h[0];
+      }
+  }
+
+  template <int DIM, typename T>
+  __device__ __forceinline__ T interp_v(T u)
+  {
+    const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    T       a  = sa * u + sb * xshfl<1>(u);
+    a          = sa * a + sb * xshfl<2>(a);
+    if constexpr (DIM == 3)
+      a = sa * a + sb * xshfl<4>(a);
+    return a;
+  }
+
+  // transpose of interp_vg: nodal contribution of (v, g) given at q-points
+  // (g already scaled by JxW/h)
+  template <int DIM, typename T>
+  __device__ __forceinline__ T integrate_vg(T v, const T *g, int corner)
+  {
+    const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    const T s0 = (corner & 1) ? T(1) : T(-1);
+    const T s1 = (corner & 2) ? T(1) : T(-1);
+    if constexpr (DIM == 2)
+      {
+        const T pv  = xshfl<2>(v);
+        const T pgy = xshfl<2>(g[1]);
+        const T pgx = xshfl<2>(g[0]);
+        const T U1  = sa * v + sb * pv + s1 * (g[1] + pgy);
+        const T Ux  = sa * g[0] + sb * pgx;
+        const T pU1 = xshfl<1>(U1);
+        const T pUx = xshfl<1>(Ux);
+        return sa * U1 + sb * pU1 + s0 * (Ux + pUx);
+      }
+    else
+      {
+        const T s2  = (corner & 4) ? T(1) : T(-1);
+        const T pv  = xshfl<4>(v);
+        const T pgz = xshfl<4>(g[2]);
+        const T pgy = xshfl<4>(g[1]);
+        const T pgx = xshfl<4>(g[0]);
+        const T T1  = sa * v + sb * pv + s2 * (g[2] + pgz);
+        const T Ty  = sa * g[1] + sb * pgy;
+        const T Tx  = sa * g[0] + sb * pgx;
+        const T pT1 = xshfl<2>(T1);
+        const T pTy = xshfl<2>(Ty);
+        const T pTx = xshfl<2>(Tx);
+        const T U1  = sa * T1 + sb * pT1 + s1 * (Ty + pTy);
+        const T Ux  = sa * Tx + sb * pTx;
+        const T pU1 = xshfl<1>(U1);
+        const T pUx = xshfl<1>(Ux);
+        return sa * U1 + sb * pU1 + s0 * (Ux + pUx);
+      }
+  }
+
+  // sum over the 2^DIM lanes of a cell
+  template <int DIM, typename T>
+  __device__ __forceinline__ T cell_sum(T v)
+  {
+    v += xshfl<1>(v);
+    v += xshfl<2>(v);
+    if constexpr (DIM == 3)
+      v += xshfl<4>(v);
+    return v;
+  }
+
+  // ------------------------------------------------------------ the tile
+  template <int DIM, int TX, int TY>
+  struct Tile
+  {
+    static constexpr int NV = 1 << DIM;
+    static constexpr int NT = TX * TY * NV;
+    static constexpr int PX = TX + 1;
+    static constexpr int PY = DIM == 3 ? TY + 1 : 1;
+    static constexpr int PN = PX * PY;
+
+    int  tid, corner, cx, cy, bx, by, bm;
+    int  x0, y0, m0, tx_eff, ty_eff, tm_eff;
+    int  pn_lane; // plane-node index of the lane's node
+    bool cell_valid;
+
+    __device__ __forceinline__ Tile(const KParams &kp)
+    {
+      tid        = threadIdx.x;
+      corner     = tid & (NV - 1);
+      const int cell = tid >> DIM;
+      cx         = cell % TX;
+      cy         = cell / TX;
+      bx         = corner & 1;
+      by         = DIM == 3 ? (corner >> 1) & 1 : 0;
+      bm         = corner >> (DIM - 1);
+      int b      = blockIdx.x;
+      const int tix = b % kp.nt[0];
+      b /= kp.nt[0];
+      const int tiy = b % kp.nt[1];
+      const int tim = b / kp.nt[1];
+      x0         = tix * TX;
+      y0         = tiy * TY;
+      m0         = tim * kp.tz;
+      tx_eff     = min(TX, kp.nc[0] - x0);
+      ty_eff     = DIM == 3 ? min(TY, kp.nc[1] - y0) : 1;
+      tm_eff     = min(kp.tz, kp.nc[2] - m0);
+      cell_valid = cx < tx_eff && cy < ty_eff;
+      pn_lane    = (cx + bx) + PX * (cy + by);
+    }
+
+    __device__ __forceinline__ bool node_in_tile(int nx, int ny) const
+    {
+      return nx <= tx_eff && (DIM == 2 || ny <= ty_eff);
+    }
+
+    __device__ __forceinline__ int64_t gnode(const KParams &kp, int nx, int ny, int k) const
+    {
+      return ((int64_t)(m0 + k) * kp.nn[1] + (y0 + ny)) * kp.nn[0] + (x0 + nx);
+    }
+
+    // node shared with another tile (its sum is finished by q1_fixup)
+    __device__ __forceinline__ bool shared(const KParams &kp, int nx, int ny, int k) const
+    {
+      bool s = (nx == 0 && x0 > 0) || (nx == tx_eff && x0 + tx_eff < kp.nc[0]) ||
+               (k == 0 && m0 > 0) || (k == tm_eff && m0 + tm_eff < kp.nc[2]);
+      if constexpr (DIM == 3)
+        s = s || (ny == 0 && y0 > 0) || (ny == ty_eff && y0 + ty_eff < kp.nc[1]);
+      return s;
+    }
+  };
+
+  // output of one node/component of plane k: the node's own entry of dst or
+  // this tile's partial
+  template <int DIM, int TX, int TY, typename T>
+  __device__ __forceinline__ void put(const KParams &kp, const Tile<DIM, TX, TY> &t, int nc,
+                                      int k, int pn, int c, T v, T *dst, T *part,
+                                      int ostride, int ocstride, int ooffset)
+  {
+    using TL     = Tile<DIM, TX, TY>;
+    const int nx = pn % TL::PX, ny = pn / TL::PX;
+    if (!t.node_in_tile(nx, ny))
+      return;
+    if (t.shared(kp, nx, ny, k))
+      part[(((int64_t)blockIdx.x * (kp.tz + 1) + k) * TL::PN + pn) * nc + c] = v;
+    else
+      dst[t.gnode(kp, nx, ny, k) * ostride + (int64_t)c * ocstride + ooffset] = v;
+  }
+
+  // ------------------------------------------- vmult / residual kernel
+  template <int DIM, int TX, int TY, int G, bool TENS, int OP, typename T>
+  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    q1_apply(const KParams kp, const T *__restrict__ src, const T *__restrict__ old0,
+             const T *__restrict__ old1, const T *__restrict__ old2,
+             const T *__restrict__ cache, T *__restrict__ dst, T *__restrict__ part)
+  {
+    using TL             = Tile<DIM, TX, TY>;
+    using CL             = CacheLayout<DIM, G, TENS>;
+    constexpr int  NC    = G + 2;
+    constexpr int  NT    = TL::NT;
+    constexpr int  NV    = TL::NV;
+    constexpr int  PN    = TL::PN;
+    constexpr bool HIST  = OP == OP_RES2;
+    constexpr int  NPL   = HIST ? 2 : 1;
+    constexpr int  ITEMS = (PN * NC + NT - 1) / NT;
+    constexpr int  NF    = CL::n_fields;
+
+    __shared__ T ring[2][NPL][PN * NC];
+    __shared__ T slots[NV][PN * NC];
+
+    const TL t(kp);
+    const int tid = t.tid;
+
+    T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+    const T jxw = T(kp.jxw);
+
+    T regs[NPL][ITEMS];
+    auto load_plane = [&](int k) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        {
+          const int i = tid + r * NT;
+          if (i < PN * NC)
+            {
+              const int pn = i / NC, c = i - pn * NC;
+              const int nx = pn % TL::PX, ny = pn / TL::PX;
+              T         s = T(0), h = T(0);
+              if (t.node_in_tile(nx, ny))
+                {
+                  const int64_t gi = t.gnode(kp, nx, ny, k) * NC + c;
+                  s                = src[gi];
+                  if constexpr (HIST)
+                    {
+                      // do_evalute_history, S/operator_sintering_generic.h:950-960
+                      h = old0[gi] * T(kp.w_hist[0]);
+                      if (kp.order >= 2)
+                        h += old1[gi] * T(kp.w_hist[1]);
+                      if (kp.order >= 3)
+                        h += old2[gi] * T(kp.w_hist[2]);
+                    }
+                }
+              regs[0][r] = s;
+              if constexpr (HIST)
+                regs[1][r] = h;
+            }
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        {
+          const int i = tid + r * NT;
+          if (i < PN * NC)
+            {
+#pragma unroll
+              for (int a = 0; a < NPL; ++a)
+                ring[slot][a][i] = regs[a][r];
+            }
+        }
+    };
+
+    T carry[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      carry[r] = T(0);
+
+    load_plane(0);
+    store_plane(0);
+    load_plane(1);
+    store_plane(1);
+    __syncthreads();
+
+    const T *cache_tile = cache + (int64_t)blockIdx.x * kp.tz * NF * NT;
+
+    for (int k = 0; k < t.tm_eff; ++k)
+      {
+        const bool pre = k + 2 <= t.tm_eff;
+        if (pre)
+          load_plane(k + 2);
+
+        // ---- cell layer k: interpolate, physics, integrate
+        T   val[NC], grad[NC][DIM];
+        const T *pl = &ring[(k + t.bm) & 1][0][t.pn_lane * NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          interp_vg<DIM>(pl[c], t.corner, inv_h, val[c], grad[c]);
+
+        T vr[NC], gr[NC][DIM];
+        if constexpr (OP == OP_VMULT)
+          {
+            T lin[NF];
+            const T *cl = cache_tile + (int64_t)k * NF * NT + tid;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+              lin[f] = cl[f * NT];
+            tangent_qp<DIM, G, TENS>(kp.pp, lin, val, grad, vr, gr);
+          }
+        else
+          {
+            T hist[NC];
+            if constexpr (HIST)
+              {
+                const T *ph = &ring[(k + t.bm) & 1][1][t.pn_lane * NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                  hist[c] = interp_v<DIM>(ph[c]);
+              }
+            residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, val, grad, hist, vr, gr);
+          }
+
+        T *sl = &slots[t.corner][t.pn_lane * NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          {
+            T gs[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              gs[d] = gr[c][d] * (jxw * inv_h[d]);
+            sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs, t.corner);
+          }
+        __syncthreads();
+
+        if (pre)
+          store_plane(k & 1);
+
+        // ---- node sums: plane k is complete, plane k+1 begins
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+          {
+            const int i = tid + r * NT;
+            if (i < PN * NC)
+              {
+                const int pn = i / NC, c = i - pn * NC;
+                const int nx = pn % TL::PX, ny = pn / TL::PX;
+                T         fin = carry[r], nxt = T(0);
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                  {
+                    const int qx = q & 1, qy = DIM == 3 ? (q >> 1) & 1 : 0;
+                    const int qm = q >> (DIM - 1);
+                    const int ccx = nx - qx, ccy = ny - qy;
+                    if (ccx >= 0 && ccx < t.tx_eff && ccy >= 0 && ccy < t.ty_eff)
+                      {
+                        const T v = slots[q][i];
+                        if (qm == 0)
+                          fin += v;
+                        else
+                          nxt += v;
+                      }
+                  }
+                carry[r] = nxt;
+                put(kp, t, NC, k, pn, c, fin, dst, part, NC, 1, 0);
+              }
+          }
+        __syncthreads();
+      }
+
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      {
+        const int i = tid + r * NT;
+        if (i < PN * NC)
+          {
+            const int pn = i / NC, c = i - pn * NC;
+            put(kp, t, NC, t.tm_eff, pn, c, carry[r], dst, part, NC, 1, 0);
+          }
+      }
+  }
+
+  // ------------------------------------------- linearisation cache fill
+  // SinteringOperatorData::fill_quadrature_point_values,
+  // S/operator_sintering_data.h:301-409 (only the fields the tangent reads)
+  template <int DIM, int TX, int TY, int G, bool TENS>
+  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    q1_fill(const KParams kp, const double *__restrict__ lin, double *__restrict__ cache64,
+            float *__restrict__ cache32)
+  {
+    using TL            = Tile<DIM, TX, TY>;
+    using CL            = CacheLayout<DIM, G, TENS>;
+    constexpr int NC    = G + 2;
+    constexpr int NT    = TL::NT;
+    constexpr int PN    = TL::PN;
+    constexpr int ITEMS = (PN * NC + NT - 1) / NT;
+    constexpr int NF    = CL::n_fields;
+
+    __shared__ double ring[2][PN * NC];
+    const TL          t(kp);
+    const int         tid      = t.tid;
+    const double      inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
+
+    double regs[ITEMS];
+    auto   load_plane = [&](int k) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        {
+          const int i = tid + r * NT;
+          if (i < PN * NC)
+            {
+              const int pn = i / NC, c = i - pn * NC;
+              const int nx = pn % TL::PX, ny = pn / TL::PX;
+              regs[r] = t.node_in_tile(nx, ny) ? lin[t.gnode(kp, nx, ny, k) * NC + c] : 0.0;
+            }
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        {
+          const int i = tid + r * NT;
+          if (i < PN * NC)
+            ring[slot][i] = regs[r];
+        }
+    };
+
+    load_plane(0);
+    store_plane(0);
+    load_plane(1);
+    store_plane(1);
+    __syncthreads();
+
+    const int64_t tile_off = (int64_t)blockIdx.x * kp.tz * NF * NT;
+    for (int k = 0; k < t.tm_eff; ++k)
+      {
+        const bool pre = k + 2 <= t.tm_eff;
+        if (pre)
+          load_plane(k + 2);
+
+        const double *pl = &ring[(k + t.bm) & 1][t.pn_lane * NC];
+        double        f[NF];
+        if constexpr (!TENS)
+          {
+            f[CL::c] = interp_v<DIM>(pl[0]);
+#pragma unroll
+            for (int i = 0; i < G; ++i)
+              f[CL::eta + i] = interp_v<DIM>(pl[2 + i]);
+            double v, g[DIM];
+            interp_vg<DIM>(pl[1], t.corner, inv_h, v, g);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              f[CL::grad_mu + d] = g[d];
+          }
+        else
+          {
+            double v, g[DIM];
+            interp_vg<DIM>(pl[0], t.corner, inv_h, v, g);
+            f[CL::c] = v;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              f[CL::grad_c + d] = g[d];
+            interp_vg<DIM>(pl[1], t.corner, inv_h, v, g);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              f[CL::grad_mu + d] = g[d];
+#pragma unroll
+            for (int i = 0; i < G; ++i)
+              {
+                interp_vg<DIM>(pl[2 + i], t.corner, inv_h, v, g);
+                f[CL::eta + i] = v;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  f[CL::grad_eta + i * DIM + d] = g[d];
+              }
+          }
+        const int64_t o = tile_off + (int64_t)k * NF * NT + tid;
+#pragma unroll
+        for (int q = 0; q < NF; ++q)
+          {
+            cache64[o + (int64_t)q * NT] = f[q];
+            if (cache32)
+              cache32[o + (int64_t)q * NT] = (float)f[q];
+          }
+        __syncthreads();
+        if (pre)
+          store_plane(k & 1);
+        __syncthreads();
+      }
+  }
+
+  // --------------------------------- diagonal / block column by probing
+  // Matrix-free assembly of the tangent's diagonal (column < 0) or of column
+  // `column` of every node's n_c x n_c block, by applying the quadrature
+  // operator to unit vectors of the cell's local dofs -- the recipe of
+  // MatrixFreeTools::compute_diagonal (used by OperatorBase::
+  // compute_inverse_diagonal, S/operator_base.h:364-389).
+  template <int DIM, int TX, int TY, int G, bool TENS>
+  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    q1_probe(const KParams kp, const double *__restrict__ cache, double *__restrict__ dst,
+             double *__restrict__ part, int column, int ostride, int ocstride, int ooffset)
+  {
+    using TL            = Tile<DIM, TX, TY>;
+    using CL            = CacheLayout<DIM, G, TENS>;
+    constexpr int NC    = G + 2;
+    constexpr int NT    = TL::NT;
+    constexpr int NV    = TL::NV;
+    constexpr int PN    = TL::PN;
+    constexpr int ITEMS = (PN * NC + NT - 1) / NT;
+    constexpr int NF    = CL::n_fields;
+    using T             = double;
+
+    __shared__ T slots[NV][PN * NC];
+    const TL     t(kp);
+    const int    tid = t.tid;
+    const T      inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
+    const T      jxw      = kp.jxw;
+    const T      sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+
+    T carry[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      carry[r] = T(0);
+
+    const T *cache_tile = cache + (int64_t)blockIdx.x * kp.tz * NF * NT;
+    for (int k = 0; k < t.tm_eff; ++k)
+      {
+        T lin[NF];
+        const T *cl = cache_tile + (int64_t)k * NF * NT + tid;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          lin[f] = cl[f * NT];
+
+        T mine[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          mine[c] = T(0);
+
+        for (int j = 0; j < NC; ++j)
+          {
+            if (column >= 0 && j != column)
+              continue;
+            for (int kk = 0; kk < NV; ++kk)
+              {
+                // shape value / gradient of local node kk at this lane's q-point
+                T S = T(1), Gd[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  {
+                    const int qd = (t.corner >> d) & 1, nd = (kk >> d) & 1;
+                    S *= (qd == nd) ? sa : sb;
+                  }
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  {
+                    T g = (((kk >> d) & 1) ? T(1) : T(-1)) * inv_h[d];
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e)
+                      if (e != d)
+                        g *= (((t.corner >> e) & 1) == ((kk >> e) & 1)) ? sa : sb;
+                    Gd[d] = g;
+                  }
+                T v[NC], g[NC][DIM];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                  {
+                    v[c] = c == j ? S : T(0);
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d)
+                      g[c][d] = c == j ? Gd[d] : T(0);
+                  }
+                T vr[NC], gr[NC][DIM];
+                tangent_qp<DIM, G, TENS>(kp.pp, lin, v, g, vr, gr);
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+                  {
+                    if (column < 0 && i != j)
+                      continue;
+                    T s = S * vr[i];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d)
+                      s += Gd[d] * gr[i][d];
+                    s = cell_sum<DIM>(jxw * s);
+                    if (t.corner == kk)
+                      mine[i] = s;
+                  }
+              }
+          }
+        T *sl = &slots[t.corner][t.pn_lane * NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          sl[c] = mine[c];
+        __syncthreads();
+
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+          {
+            const int i = tid + r * NT;
+            if (i < PN * NC)
+              {
+                const int pn = i / NC, c = i - pn * NC;
+                const int nx = pn % TL::PX, ny = pn / TL::PX;
+                T         fin = carry[r], nxt = T(0);
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                  {
+                    const int qx = q & 1, qy = DIM == 3 ? (q >> 1) & 1 : 0;
+                    const int qm = q >> (DIM - 1);
+                    const int ccx = nx - qx, ccy = ny - qy;
+                    if (ccx >= 0 && ccx < t.tx_eff && ccy >= 0 && ccy < t.ty_eff)
+                      {
+                        const T val = slots[q][i];
+                        if (qm == 0)
+                          fin += val;
+                        else
+                          nxt += val;
+                      }
+                  }
+                carry[r] = nxt;
+                put(kp, t, NC, k, pn, c, fin, dst, part, ostride, ocstride, ooffset);
+              }
+          }
+        __syncthreads();
+      }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      {
+        const int i = tid + r * NT;
+        if (i < PN * NC)
+          {
+            const int pn = i / NC, c = i - pn * NC;
+            put(kp, t, NC, t.tm_eff, pn, c, carry[r], dst, part, ostride, ocstride, ooffset);
+          }
+      }
+  }
+
+  // ------------------------------------------- shared-node fix-up
+  struct FixParams
+  {
+    int nc[3], nn[3], nt[3], tsz[3]; // tile sizes: TX, TY, tz
+    int n_comp;
+    int px, pn;                      // plane-node geometry of the tile
+    int nb[3];                       // interior tile boundaries per direction
+    int64_t n_m, n_y, n_x;           // thread counts of the three classes
+    int ostride, ocstride, ooffset;
+  };
+
+  __device__ __forceinline__ bool is_bnd(int g, int tsz, int nc)
+  {
+    return g > 0 && g < nc && (g % tsz) == 0;
+  }
+
+  template <typename T>
+  __global__ void __launch_bounds__(256)
+    q1_fixup(const FixParams fp, const T *__restrict__ part, T *__restrict__ dst)
+  {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int     g[3];
+    // class M: planes on march boundaries (all x, y)
+    if (idx < fp.n_m)
+      {
+        g[0] = (int)(idx % fp.nn[0]);
+        idx /= fp.nn[0];
+        g[1] = (int)(idx % fp.nn[1]);
+        g[2] = (int)(idx / fp.nn[1] + 1) * fp.tsz[2];
+      }
+    else if ((idx -= fp.n_m) < fp.n_y)
+      {
+        g[0] = (int)(idx % fp.nn[0]);
+        idx /= fp.nn[0];
+        g[2] = (int)(idx % fp.nn[2]);
+        g[1] = (int)(idx / fp.nn[2] + 1) * fp.tsz[1];
+        if (is_bnd(g[2], fp.tsz[2], fp.nc[2]))
+          return;
+      }
+    else if ((idx -= fp.n_y) < fp.n_x)
+      {
+        g[1] = (int)(idx % fp.nn[1]);
+        idx /= fp.nn[1];
+        g[2] = (int)(idx % fp.nn[2]);
+        g[0] = (int)(idx / fp.nn[2] + 1) * fp.tsz[0];
+        if (is_bnd(g[2], fp.tsz[2], fp.nc[2]) || is_bnd(g[1], fp.tsz[1], fp.nc[1]))
+          return;
+      }
+    else
+      return;
+
+    // sharing tiles and local indices along each direction
+    int tl[3][2], lo[3][2], ns[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      {
+        if (is_bnd(g[d], fp.tsz[d], fp.nc[d]))
+          {
+            ns[d]    = 2;
+            tl[d][0] = g[d] / fp.tsz[d] - 1;
+            lo[d][0] = fp.tsz[d];
+            tl[d][1] = g[d] / fp.tsz[d];
+            lo[d][1] = 0;
+          }
+        else
+          {
+            ns[d]    = 1;
+            tl[d][0] = min(g[d] / fp.tsz[d], fp.nt[d] - 1);
+            lo[d][0] = g[d] - tl[d][0] * fp.tsz[d];
+          }
+      }
+    const int64_t node = ((int64_t)g[2] * fp.nn[1] + g[1]) * fp.nn[0] + g[0];
+    for (int c = 0; c < fp.n_comp; ++c)
+      {
+        T s = T(0);
+        for (int a = 0; a < ns[2]; ++a)
+          for (int b = 0; b < ns[1]; ++b)
+            for (int e = 0; e < ns[0]; ++e)
+              {
+                const int64_t tile =
+                  ((int64_t)tl[2][a] * fp.nt[1] + tl[1][b]) * fp.nt[0] + tl[0][e];
+                const int pn = lo[0][e] + fp.px * lo[1][b];
+                s += part[((tile * (fp.tsz[2] + 1) + lo[2][a]) * fp.pn + pn) * fp.n_comp + c];
+              }
+        dst[node * fp.ostride + (int64_t)c * fp.ocstride + fp.ooffset] = s;
+      }
+  }
+} // namespace hps

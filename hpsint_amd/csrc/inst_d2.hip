@@ -1,0 +1,54 @@
+// Instantiations: 2D Q1, scalar and tensorial mobility; the fix-up kernel.
+#include "ops.hpp"
+
+namespace hps
+{
+  const OpTable *q1_table_d3_scalar(int n_grains);
+  const OpTable *q1_table_d3_tensorial(int n_grains);
+
+  const OpTable *q1_table_d3(int n_grains, bool tensorial)
+  {
+    return tensorial ? q1_table_d3_tensorial(n_grains) : q1_table_d3_scalar(n_grains);
+  }
+
+  const OpTable *q1_table_d2(int n_grains, bool tensorial)
+  {
+#define HPS_CASE(G, TENS)                                                  \
+  case G:                                                                  \
+    {                                                                      \
+      static const OpTable t = Q1Launch<2, TX2, TY2, G, TENS>::table();    \
+      return &t;                                                           \
+    }
+    if (!tensorial)
+      switch (n_grains)
+        {
+          HPS_CASE(1, false)
+          HPS_CASE(2, false)
+          HPS_CASE(3, false)
+          HPS_CASE(4, false)
+          default:
+            return nullptr;
+        }
+    switch (n_grains)
+      {
+        HPS_CASE(1, true)
+        HPS_CASE(2, true)
+        HPS_CASE(3, true)
+        default:
+          return nullptr;
+      }
+#undef HPS_CASE
+  }
+
+  template <typename T>
+  void launch_fixup(const FixParams &fp, const T *part, T *dst, hipStream_t s)
+  {
+    const int64_t n = fp.n_m + fp.n_y + fp.n_x;
+    if (n == 0)
+      return;
+    const int blocks = (int)((n + 255) / 256);
+    q1_fixup<T><<<blocks, 256, 0, s>>>(fp, part, dst);
+  }
+  template void launch_fixup<double>(const FixParams &, const double *, double *, hipStream_t);
+  template void launch_fixup<float>(const FixParams &, const float *, float *, hipStream_t);
+} // namespace hps

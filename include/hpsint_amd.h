@@ -1,0 +1,177 @@
+/*
+ * hpsint_amd.h -- C ABI of the MI355X sintering-operator library.
+ *
+ * Drop-in for the hot path of hpsint's SinteringOperatorGeneric: residual,
+ * tangent vmult, linearisation cache, block-vector BLAS-1 and Jacobi
+ * preconditioners.  All vector arguments are DEVICE pointers (HBM-resident)
+ * in the component-interleaved layout  x[node * n_components + component]
+ * (component 0 = c, 1 = mu, 2.. = eta_i; nodes lexicographic, x fastest),
+ * i.e. the layout of hpsint's VectorTools::merge_components_fast
+ * (include/pf-applications/numerics/vector_tools.h:40-100).
+ * hps_to_blocks / hps_from_blocks convert to and from the one-vector-per-
+ * component layout of DynamicBlockVector.
+ *
+ * Every call is enqueued on the operator's stream (hps_set_stream) and
+ * returns without waiting, except the reductions that return a value to the
+ * host (hps_dot, hps_norm2, hps_multi_dot, hps_sync).  Calls on one handle
+ * must not overlap from several host threads.
+ *
+ * Return value: HPS_OK or one of the error codes below; hps_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Reference interfaces are cited as
+ *   S/ = applications/sintering/include/pf-applications/sintering/
+ *   I/ = include/pf-applications/
+ */
+#ifndef HPSINT_AMD_H
+#define HPSINT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+  HPS_OK                    = 0,
+  HPS_ERR_INVALID_ARGUMENT  = 1,
+  HPS_ERR_INVALID_N_COMP    = 2, /* ExcInvalidNumberOfComponents, S/instantiation.h:61-68 */
+  HPS_ERR_UNSUPPORTED       = 3, /* dim / degree / mobility not built */
+  HPS_ERR_OUT_OF_MEMORY     = 4,
+  HPS_ERR_HIP               = 5,
+  HPS_ERR_NOT_INITIALIZED   = 6  /* vmult before set_linearization etc. */
+};
+
+enum
+{
+  HPS_MOBILITY_SCALAR    = 0, /* MobilityScalar, S/mobility.h:247-568 */
+  HPS_MOBILITY_TENSORIAL = 1  /* MobilityTensorial, S/mobility.h:573-961 */
+};
+
+enum
+{
+  HPS_PRECON_POINT_JACOBI      = 0, /* InverseDiagonalMatrix, I/lac/preconditioners.h:160-215 */
+  HPS_PRECON_NODE_BLOCK_JACOBI = 1  /* InverseComponentBlockDiagonalMatrix, :218-318 */
+};
+
+typedef struct hps_operator_s *hps_operator;
+
+/* Mesh, discretisation and material of one operator.  Mirrors the
+ * parameters the reference reads from Material.EnergyAbstract.* and
+ * Material.MobilityAbstract.* (S/parameters.h:113-134) and the uniform
+ * Cartesian mesh of the "Divisions" mode (S/driver.h:575-590). */
+typedef struct
+{
+  int    dim;           /* 2 or 3 */
+  int    fe_degree;     /* 1 (FE_Q(1), QGauss(2)) */
+  int    n_cells[3];    /* cells per direction */
+  double cell_size[3];  /* h per direction */
+  int    n_components;  /* n_grains + 2 */
+  int    mobility;      /* HPS_MOBILITY_* */
+  double A, B, kappa_c, kappa_p;
+  double Mvol, Mvap, Msurf, Mgb, L;
+} hps_params;
+
+/* Replaces the construction of SinteringOperatorGeneric
+ * (S/operator_sintering_generic.h:341-364) together with its MatrixFree
+ * and SinteringOperatorData (S/operator_sintering_data.h:33-60). */
+int hps_create(const hps_params *params, hps_operator *op);
+int hps_destroy(hps_operator op);
+const char *hps_last_error(void);
+
+/* Stream the operator enqueues on (a hipStream_t; NULL = default stream). */
+int hps_set_stream(hps_operator op, void *hip_stream);
+int hps_sync(hps_operator op);
+
+/* n_components() / initialize_dof_vector() sizes,
+ * S/operator_sintering_generic.h:616-620, S/operator_base.h:143-149 */
+int hps_sizes(hps_operator op, int64_t *n_nodes, int64_t *n_dofs);
+
+/* TimeIntegratorData weights: get_primary_weight() = w0,
+ * get_algebraic_weight() = w_alg, get_weights()[1..order] = w_hist
+ * (I/time_integration/time_integrator_data.h:152-170). */
+int hps_set_time_weights(hps_operator op, double w0, double w_alg,
+                         const double *w_hist, int order);
+
+/* SinteringOperatorData::fill_quadrature_point_values
+ * (S/operator_sintering_data.h:301-409): builds the linearisation cache from
+ * the Newton iterate u (FP64 and FP32 copies). */
+int hps_set_linearization(hps_operator op, const double *u);
+
+/* SinteringOperatorGeneric::evaluate_nonlinear_residual<wtd>
+ * (S/operator_sintering_generic.h:393-448).  old[k] (k < order) are device
+ * pointers of the old solutions (only read for wtd == 2); dst is
+ * overwritten. */
+int hps_residual(hps_operator op, int wtd, double *dst, const double *src,
+                 const double *const *old);
+
+/* OperatorBase::vmult -> SinteringOperatorGeneric::vmult_internal
+ * (S/operator_base.h:294-353, S/operator_sintering_generic.h:579-606):
+ * tangent at the linearisation point; dst is overwritten.  The f32 variant
+ * uses the FP32 copy of the cache and FP32 arithmetic throughout. */
+int hps_vmult_f64(hps_operator op, double *dst, const double *src);
+int hps_vmult_f32(hps_operator op, float *dst, const float *src);
+
+/* Preconditioner setup (PreconditionerBase::do_update,
+ * I/lac/preconditioners.h:77-112): kind = HPS_PRECON_POINT_JACOBI computes
+ * the inverse diagonal as OperatorBase::compute_inverse_diagonal does
+ * (S/operator_base.h:364-389: 1/d if |d| > 1e-10 else 1);
+ * HPS_PRECON_NODE_BLOCK_JACOBI inverts every node's n_c x n_c block
+ * (InverseComponentBlockDiagonalMatrix::do_update, Gauss-Jordan). */
+int hps_precond_update(hps_operator op, int kind);
+/* PreconditionerBase::vmult */
+int hps_precond_apply_f64(hps_operator op, double *dst, const double *src);
+int hps_precond_apply_f32(hps_operator op, float *dst, const float *src);
+/* copies of the preconditioner data for inspection: the inverse diagonal
+ * (n_dofs entries) or the node blocks before inversion
+ * (n_nodes * n_c * n_c, row-major per node) */
+int hps_precond_get_diagonal(hps_operator op, double *inv_diag);
+int hps_precond_get_blocks(hps_operator op, double *blocks);
+
+/* DynamicBlockVector BLAS-1 (I/lac/dynamic_block_vector.h:346-537) on whole
+ * interleaved vectors of n entries.  Reductions write one double to host
+ * memory and wait for it. */
+int hps_dot(hps_operator op, int64_t n, const double *x, const double *y,
+            double *result);
+int hps_norm2(hps_operator op, int64_t n, const double *x, double *result);
+/* per-component l2 norms (the block norms DynamicBlockVector::block(b).
+ * l2_norm() would give): n_components results */
+int hps_block_norms(hps_operator op, const double *x, double *result);
+int hps_axpy(hps_operator op, int64_t n, double a, const double *x, double *y);
+/* y = s*y + a*x  (sadd) */
+int hps_sadd(hps_operator op, int64_t n, double s, double a, const double *x,
+             double *y);
+int hps_scale(hps_operator op, int64_t n, double a, double *x);
+int hps_copy(hps_operator op, int64_t n, double *dst, const double *src);
+int hps_fill(hps_operator op, int64_t n, double value, double *x);
+/* classical Gram-Schmidt step of GMRES: h[j] = <V_j, w> for j < k, with the
+ * basis V stored as k contiguous vectors of n entries (V + j*n) */
+int hps_multi_dot(hps_operator op, int64_t n, int k, const double *V,
+                  const double *w, double *h);
+/* w -= sum_j h[j] V_j, then returns ||w|| (fused CGS update + norm) */
+int hps_multi_axpy_norm(hps_operator op, int64_t n, int k, const double *V,
+                        const double *h, double *w, double *norm);
+/* conversions FP64 <-> FP32 */
+int hps_convert_f64_to_f32(hps_operator op, int64_t n, float *dst,
+                           const double *src);
+int hps_convert_f32_to_f64(hps_operator op, int64_t n, double *dst,
+                           const float *src);
+
+/* layout conversion: merge_components_fast / split_up_components_fast
+ * (I/numerics/vector_tools.h:40-100); blocks is a host array of n_c device
+ * pointers of n_nodes entries each */
+int hps_from_blocks(hps_operator op, double *x, const double *const *blocks);
+int hps_to_blocks(hps_operator op, double *const *blocks, const double *x);
+
+/* duration in milliseconds of the last hps_vmult_* / hps_residual call as
+ * measured by HIP events on the operator's stream around its kernels
+ * (main kernel + shared-node fix-up); for measurement only */
+int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fixup);
+int hps_enable_timing(hps_operator op, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

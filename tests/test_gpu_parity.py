@@ -1,0 +1,236 @@
+"""HIP path vs the CPU oracle, through the C ABI, on the same seeded inputs.
+
+Bar: FP64 relative l2 error <= 1e-12 (residual, vmult, diagonal, node
+blocks); bitwise-identical repeats; FP32 vmult within the per-field bound
+written in the test.  Meshes are ragged (not multiples of the tile) so that
+partial tiles and every class of shared node are exercised.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,  # noqa: E402
+                        Preconditioner, SinteringOperatorGeneric)
+from hpsint_amd import _ffi  # noqa: E402
+
+TOL64 = 1e-12
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def dev(x, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device="cuda", dtype=dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def make_pair(dim, cells, G, tensorial=False, h=None, fe=None, mob=None, w0=0.0,
+              w_hist=()):
+    h = h or (0.5, 0.75, 1.0)[:dim]
+    fe = fe or FreeEnergyParams()
+    mob = mob or MobilityParams(tensorial=tensorial)
+    mesh = CartesianMesh(tuple(cells), (0.0,) * dim,
+                         tuple(n * hh for n, hh in zip(cells, h)))
+    op = SinteringOperatorGeneric(mesh, G, fe, mob)
+    op.set_time_weights(w0, w_hist)
+    p = po.make_params(dim, 1, cells, h, G + 2, tensorial=tensorial, A=fe.A, B=fe.B,
+                       kappa_c=fe.kappa_c, kappa_p=fe.kappa_p, Mvol=mob.Mvol,
+                       Mvap=mob.Mvap, Msurf=mob.Msurf, Mgb=mob.Mgb, L=mob.L, w0=w0,
+                       w_hist=w_hist)
+    return op, p
+
+
+def fields(p, seed):
+    """random nodal state: c, eta in [0,1], mu in [-1,1]"""
+    rng = np.random.default_rng(seed)
+    nn = po.n_nodes(p)
+    u = rng.uniform(0.0, 1.0, size=(nn, p.n_comp))
+    u[:, 1] = rng.uniform(-1.0, 1.0, size=nn)
+    return u.ravel()
+
+
+VMULT_CASES = [
+    (3, (9, 5, 17), 2, False),
+    (3, (19, 13, 37), 4, False),
+    (3, (8, 4, 16), 1, False),
+    (3, (3, 7, 2), 6, False),
+    (3, (11, 6, 18), 3, True),
+    (3, (5, 9, 20), 2, True),
+    (2, (65, 17), 2, False),
+    (2, (130, 33), 4, False),
+    (2, (70, 19), 2, True),
+]
+
+
+@pytest.mark.parametrize("dim,cells,G,tens", VMULT_CASES)
+def test_vmult_f64_matches_oracle(dim, cells, G, tens):
+    op, p = make_pair(dim, cells, G, tens, w0=100.0)
+    u = fields(p, 1)
+    src = np.random.default_rng(2).uniform(-1, 1, size=u.size)
+    cache = po.fill_qp(p, u)
+    ref = po.vmult(p, cache, src)
+    du = dev(u)
+    op.fill_quadrature_point_values(du)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, dev(src))
+    got = host(dst)
+    assert rel_l2(got, ref) <= TOL64, rel_l2(got, ref)
+    # deterministic: a second application is bitwise identical
+    dst2 = op.initialize_dof_vector()
+    op.vmult(dst2, dev(src))
+    assert np.array_equal(host(dst2), got)
+
+
+RES_CASES = [
+    (3, (9, 5, 17), 2, False),
+    (3, (19, 13, 37), 3, False),
+    (3, (11, 6, 18), 3, True),
+    (2, (65, 17), 2, False),
+    (2, (70, 19), 2, True),
+]
+
+
+@pytest.mark.parametrize("wtd", [0, 1, 2])
+@pytest.mark.parametrize("dim,cells,G,tens", RES_CASES)
+def test_residual_matches_oracle(dim, cells, G, tens, wtd):
+    # BDF2 with unequal steps (time_schemes.h:87-105)
+    dt0, dt1 = 1e-3, 2e-3
+    w = [(2 * dt0 + dt1) / (dt0 * (dt0 + dt1)), -(dt0 + dt1) / (dt0 * dt1),
+         dt0 / (dt1 * (dt0 + dt1))]
+    op, p = make_pair(dim, cells, G, tens, w0=w[0], w_hist=(w[1], w[2]))
+    u = fields(p, 3)
+    old = [fields(p, 4), fields(p, 5)]
+    ref = po.residual(p, wtd, u, old)
+    dst = op.initialize_dof_vector()
+    op.evaluate_nonlinear_residual(dst, dev(u), wtd, old=[dev(o) for o in old])
+    got = host(dst)
+    assert rel_l2(got, ref) <= TOL64, rel_l2(got, ref)
+
+
+def test_residual_golden_2cell_on_gpu(golden_dir):
+    """tests/residual_generic_{scalar,tensorial}_no_rbm.mpirun=1.output"""
+    import json
+    import os
+    fx = json.load(open(os.path.join(golden_dir, "residual_generic_2cell.json")))
+    s = fx["setup"]
+    u = np.zeros((6, 4))
+    for key, vals in s["nodal_values"].items():
+        x, y = map(int, key.split(","))
+        u[x + 3 * y] = vals
+    perm = [pt[0] + 3 * pt[1] for pt in s["dof_points"]]
+    for kind in ("scalar", "tensorial"):
+        fe = FreeEnergyParams(A=s["A"], B=s["B"], kappa_c=s["kappa_c"], kappa_p=s["kappa_p"])
+        mob = MobilityParams(Mvol=s["Mvol"], Mvap=s["Mvap"], Msurf=s["Msurf"], Mgb=s["Mgb"],
+                             L=s["L"], tensorial=kind == "tensorial")
+        mesh = CartesianMesh((2, 1), (0.0, 0.0), (2.0, 1.0))
+        op = SinteringOperatorGeneric(mesh, 2, fe, mob)
+        op.set_time_weights(1.0 / s["dt"], (-1.0 / s["dt"],))
+        dst = op.initialize_dof_vector()
+        du = dev(u.ravel())
+        op.evaluate_nonlinear_residual(dst, du, 2, old=[du])
+        r = host(dst).reshape(6, 4)
+        for b in range(4):
+            got = [float("%.3e" % r[perm[i], b]) for i in range(6)]
+            np.testing.assert_allclose(got, fx[kind][b], rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("dim,cells,G,tens", [
+    (3, (9, 5, 17), 2, False), (3, (5, 9, 20), 2, True), (2, (65, 17), 3, False)])
+def test_vmult_f32_error_per_field(dim, cells, G, tens):
+    """FP32 vmult (FP32 cache, FP32 arithmetic) against the FP64 oracle;
+    bound: per-field relative l2 error <= 1e-5"""
+    op, p = make_pair(dim, cells, G, tens, w0=100.0)
+    u = fields(p, 1)
+    src = np.random.default_rng(2).uniform(-1, 1, size=u.size)
+    ref = po.vmult(p, po.fill_qp(p, u), src).reshape(-1, p.n_comp)
+    op.fill_quadrature_point_values(dev(u))
+    dst = op.initialize_dof_vector(torch.float32)
+    op.vmult(dst, dev(src, torch.float32))
+    got = host(dst).reshape(-1, p.n_comp)
+    errs = [rel_l2(got[:, c], ref[:, c]) for c in range(p.n_comp)]
+    assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("dim,cells,G,tens", [
+    (3, (9, 5, 17), 2, False), (3, (5, 9, 20), 2, True), (2, (65, 17), 2, False)])
+def test_diagonal_and_node_blocks_match_oracle(dim, cells, G, tens):
+    op, p = make_pair(dim, cells, G, tens, w0=1000.0)
+    u = fields(p, 6)
+    cache = po.fill_qp(p, u)
+    blocks_ref = po.node_blocks(p, cache)
+    op.fill_quadrature_point_values(dev(u))
+    blocks = host(op.node_blocks()).reshape(blocks_ref.shape)
+    assert rel_l2(blocks, blocks_ref) <= TOL64
+    inv = op.initialize_dof_vector()
+    op.compute_inverse_diagonal(inv)
+    d = np.einsum("nii->ni", blocks_ref).ravel()
+    inv_ref = np.where(np.abs(d) > 1e-10, 1.0 / d, 1.0)
+    assert rel_l2(host(inv), inv_ref) <= TOL64
+    # block-Jacobi apply = solve with each node block
+    P = Preconditioner(op, "InverseComponentBlockDiagonalMatrix")
+    P.do_update()
+    x = np.random.default_rng(9).standard_normal(u.size)
+    y = op.initialize_dof_vector()
+    P.vmult(y, dev(x))
+    yr = np.linalg.solve(blocks_ref, x.reshape(-1, p.n_comp)[..., None])[..., 0].ravel()
+    assert rel_l2(host(y), yr) <= 1e-11
+
+
+def test_blas1_and_layout_conversions():
+    mesh = CartesianMesh((7, 5, 3), (0, 0, 0), (7.0, 5.0, 3.0))
+    op = SinteringOperatorGeneric(mesh, 3)
+    h = op.handle
+    L = _ffi.lib()
+    n = op.n_dofs
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    dx, dy = dev(x), dev(y)
+    r = ctypes.c_double()
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    _ffi.check(L.hps_dot(h, n, P(dx), P(dy), ctypes.byref(r)))
+    assert abs(r.value - x @ y) <= 1e-12 * abs(x) @ abs(y)
+    _ffi.check(L.hps_norm2(h, n, P(dx), ctypes.byref(r)))
+    assert abs(r.value - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+    bn = (ctypes.c_double * 5)()
+    _ffi.check(L.hps_block_norms(h, P(dx), bn))
+    np.testing.assert_allclose(list(bn), np.linalg.norm(x.reshape(-1, 5), axis=0), rtol=1e-13)
+    _ffi.check(L.hps_axpy(h, n, 2.5, P(dx), P(dy)))
+    np.testing.assert_allclose(host(dy), y + 2.5 * x, rtol=1e-15, atol=1e-15)
+    _ffi.check(L.hps_sadd(h, n, -0.5, 3.0, P(dx), P(dy)))
+    np.testing.assert_allclose(host(dy), -0.5 * (y + 2.5 * x) + 3.0 * x, rtol=1e-14, atol=1e-14)
+    _ffi.check(L.hps_scale(h, n, 4.0, P(dx)))
+    np.testing.assert_allclose(host(dx), 4.0 * x, rtol=0, atol=0)
+    # multi-dot and fused CGS update
+    k = 7
+    V = rng.standard_normal((k, n))
+    w = rng.standard_normal(n)
+    dV, dw = dev(V.ravel()), dev(w)
+    hh = (ctypes.c_double * k)()
+    _ffi.check(L.hps_multi_dot(h, n, k, P(dV), P(dw), hh))
+    np.testing.assert_allclose(list(hh), V @ w, rtol=1e-12, atol=1e-12)
+    _ffi.check(L.hps_multi_axpy_norm(h, n, k, P(dV), hh, P(dw), ctypes.byref(r)))
+    wr = w - V.T @ (V @ w)
+    np.testing.assert_allclose(host(dw), wr, rtol=1e-11, atol=1e-11)
+    assert abs(r.value - np.linalg.norm(wr)) <= 1e-11 * np.linalg.norm(wr)
+    # layout conversion
+    blocks = [torch.empty(op.n_nodes, dtype=torch.float64, device="cuda") for _ in range(5)]
+    arr = (ctypes.c_void_p * 5)(*[b.data_ptr() for b in blocks])
+    _ffi.check(L.hps_to_blocks(h, arr, P(dev(x))))
+    for c in range(5):
+        np.testing.assert_array_equal(host(blocks[c]), x.reshape(-1, 5)[:, c])
+    back = op.initialize_dof_vector()
+    _ffi.check(L.hps_from_blocks(h, P(back), arr))
+    np.testing.assert_array_equal(host(back), x)
