@@ -1,0 +1,202 @@
+"""Benchmark of the sintering-operator hot path on one MI355X.
+
+Workload (default): 3D 2x2x2 packing of 8 spheres, R = 10 (1 + U(-0.1,0.1)),
+contact overlap 0.5, Q1 on a uniform 128^3 hex mesh (2.1M cells), 4 order
+parameters -> 6 components (12.9M DoF), A=16, B=1, kappa_c=1, kappa_p=0.5,
+Mvol=1e-2, Mvap=1e-10, Msurf=4, Mgb=0.4, L=1, BDF1 dt=1e-3.
+
+A "step" is one FP64 tangent vmult (hps_vmult_f64) with the linearisation
+cache filled at the initial fields and src = U(-1,1) (seed 1234), inputs
+resident in HBM.  value = DoF * steps / time in GDoF/s.  Also reported: the
+FP64 residual (BDF1 history) and the FP32 vmult, the dominant kernel's
+roofline from HIP events on the operator's stream, and the CPU oracle timed
+on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="budget of the CPU-oracle sample (0 = skip)")
+    ap.add_argument("--cells", type=int, default=128)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,
+                            SinteringOperatorGeneric, packing)
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n = args.cells
+    cells = (n, n, n)
+    pk = packing.packing_cfg1(seed=0)
+    G = pk.n_order_params
+    mesh = CartesianMesh(cells, pk.lower, pk.upper)
+    fe = FreeEnergyParams(A=16.0, B=1.0, kappa_c=1.0, kappa_p=0.5)
+    mob = MobilityParams(Mvol=1e-2, Mvap=1e-10, Msurf=4.0, Mgb=0.4, L=1.0)
+    op = SinteringOperatorGeneric(mesh, G, fe, mob)
+    dt = 1e-3
+    op.set_time_weights(1.0 / dt, (-1.0 / dt,))
+
+    u0 = packing.initial_fields(pk, cells)
+    nc = G + 2
+    n_dofs = op.n_dofs
+    n_nodes = op.n_nodes
+    n_cells = n ** 3
+    u = torch.from_numpy(u0).to(dev)
+    src = torch.from_numpy(np.random.default_rng(1234).uniform(-1, 1, n_dofs)).to(dev)
+    dst = op.initialize_dof_vector()
+    op.fill_quadrature_point_values(u)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    # ---- headline: FP64 tangent vmult
+    for _ in range(args.warmup):
+        op.vmult(dst, src)
+    t_vmult = timed(lambda: op.vmult(dst, src), args.steps)
+
+    # ---- dominant kernel duration (HIP events on the operator's stream)
+    op.enable_timing(True)
+    mains, fixes = [], []
+    for _ in range(args.steps):
+        op.vmult(dst, src)
+        a, b = op.last_kernel_ms()
+        mains.append(a)
+        fixes.append(b)
+    op.enable_timing(False)
+    t_main = float(np.mean(mains)) * 1e-3
+    t_fix = float(np.mean(fixes)) * 1e-3
+
+    # ---- residual (BDF1 history) and FP32 vmult
+    old = [u.clone()]
+    r = op.initialize_dof_vector()
+    for _ in range(args.warmup):
+        op.evaluate_nonlinear_residual(r, u, 2, old=old)
+    t_res = timed(lambda: op.evaluate_nonlinear_residual(r, u, 2, old=old), args.steps)
+    op.enable_timing(True)
+    res_main = []
+    for _ in range(max(10, args.steps // 5)):
+        op.evaluate_nonlinear_residual(r, u, 2, old=old)
+        res_main.append(op.last_kernel_ms()[0])
+    op.enable_timing(False)
+    src32 = src.float()
+    dst32 = op.initialize_dof_vector(torch.float32)
+    for _ in range(args.warmup):
+        op.vmult(dst32, src32)
+    t_vm32 = timed(lambda: op.vmult(dst32, src32), args.steps)
+
+    # ---- roofline of the vmult main kernel: algorithmic bytes per launch
+    nq = 8
+    nf = 1 + G + 3          # cached fields per q-point (c, eta_i, grad mu)
+    s = 8
+    bytes_vmult = s * (2 * nc * n_nodes + n_cells * nq * nf)
+    bytes_vmult_ref = s * (2 * nc * n_nodes + n_cells * nq * (nc + 2 * 3))
+    achieved = bytes_vmult / t_main / 1e9
+    bytes_res = 8 * 3 * nc * n_nodes
+    res_achieved = bytes_res / (float(np.mean(res_main)) * 1e-3) / 1e9
+
+    # ---- CPU oracle on the host cores (bounded sample of the same workload)
+    cpu = None
+    if args.cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        p = po.make_params(3, 1, cells, mesh.cell_size, nc, A=16.0, B=1.0, kappa_c=1.0,
+                           kappa_p=0.5, Mvol=1e-2, Mvap=1e-10, Msurf=4.0, Mgb=0.4, L=1.0,
+                           w0=1.0 / dt, w_hist=(-1.0 / dt,))
+        cache = po.fill_qp(p, u0)
+        src_h = src.cpu().numpy()
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            po.vmult(p, cache, src_h)
+            reps += 1
+            if time.perf_counter() - t0 > args.cpu_seconds or reps >= 100:
+                break
+        t_cpu = (time.perf_counter() - t0) / reps
+        del cache
+        cpu = {"value": n_dofs / t_cpu / 1e9, "unit": "GDoF/s",
+               "cores": po.max_threads(), "kind": "port",
+               "sample": "%d FP64 tangent vmults of the full 128^3 workload by the C "
+                         "oracle (direct per-cell quadrature, OpenMP over colour classes; "
+                         "CPU restatement, not deal.II) on %s" % (reps, _cpu_model())}
+
+    out = {
+        "metric": "tangent vmult GDoF/s (% HBM roofline); s/timestep FP64 vs MP+learned",
+        "value": n_dofs / t_vmult / 1e9,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_vmult * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D 2x2x2 packing of 8 spheres, Q1 %d^3 hex mesh, %d order "
+                        "parameters (%d components, %d DoF); FP64 tangent vmult, "
+                        "QP-cache variant (cache: c, eta_i, grad mu per q-point)"
+                        % (n, G, nc, n_dofs),
+            "n_cells": n_cells, "n_dofs": n_dofs, "n_components": nc,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "kernel_ms": {"vmult_main": t_main * 1e3, "vmult_fixup": t_fix * 1e3,
+                      "residual_main": float(np.mean(res_main))},
+        "bytes_per_vmult": bytes_vmult,
+        "bytes_per_vmult_reference_cache": bytes_vmult_ref,
+        "residual_gdofs": n_dofs / t_res / 1e9,
+        "residual_hbm_frac": res_achieved / HBM_PEAK_GBS,
+        "vmult_f32_gdofs": n_dofs / t_vm32 / 1e9,
+    }
+    print(json.dumps(out))
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+if __name__ == "__main__":
+    main()
