@@ -8,7 +8,8 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhpsint_amd.so")
+# HPSINT_AMD_LIBRARY selects another build of the same library (measurements)
+LIB_PATH = os.environ.get("HPSINT_AMD_LIBRARY", os.path.join(HERE, "libhpsint_amd.so"))
 HEADER = os.path.join(os.path.dirname(HERE), "include", "hpsint_amd.h")
 
 HPS_OK = 0

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -39,7 +40,14 @@ namespace
     }                                                                              \
   while (0)
 
-  constexpr int TZ = 16; // cells per tile along the march direction
+  // tile geometry: cells along y per workgroup (3D) and cells per tile along
+  // the march direction; HPSINT_AMD_TILE_Y / HPSINT_AMD_TILE_M override the
+  // defaults for measurements
+  int env_int(const char *name, int dflt)
+  {
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+  }
 } // namespace
 
 struct hps_operator_s
@@ -50,7 +58,7 @@ struct hps_operator_s
   const hps::OpTable  *ops;
   hps::KParams         kp;
   hps::FixParams       fp;
-  int                  n_tiles;
+  int                  n_tiles, tz;
   int64_t              n_nodes, n_dofs, cache_elems, part_elems;
   hipStream_t          stream;
   double              *cache64;
@@ -99,7 +107,12 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   const bool tens = prm->mobility == HPS_MOBILITY_TENSORIAL;
   if (G < 1)
     return fail(HPS_ERR_INVALID_N_COMP, "n_components must be >= 3");
-  const hps::OpTable *ops = prm->dim == 3 ? hps::q1_table_d3(G, tens) : hps::q1_table_d2(G, tens);
+  const int tile_y = env_int("HPSINT_AMD_TILE_Y", 4);
+  const int TZ     = env_int("HPSINT_AMD_TILE_M", 16);
+  if (TZ < 1 || TZ > 4096)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_TILE_M out of range");
+  const hps::OpTable *ops =
+    prm->dim == 3 ? hps::q1_table_d3(G, tens, tile_y) : hps::q1_table_d2(G, tens);
   if (!ops)
     return fail(HPS_ERR_INVALID_N_COMP,
                 "n_components = " + std::to_string(prm->n_components) +
@@ -155,6 +168,7 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   const int px = ops->tx + 1;
   const int pn = px * (prm->dim == 3 ? ops->ty + 1 : 1);
   op->cache_elems = (int64_t)op->n_tiles * TZ * ops->n_fields * ops->nt;
+  op->tz          = TZ;
   op->part_elems  = (int64_t)op->n_tiles * (TZ + 1) * pn * op->nc;
 
   hps::FixParams &fp = op->fp;
@@ -208,26 +222,26 @@ extern "C" int hps_destroy(hps_operator op)
   if (!op)
     return HPS_OK;
   (void)hipStreamSynchronize(op->stream);
-  hipFree(op->cache64);
-  hipFree(op->cache32);
-  hipFree(op->part);
-  hipFree(op->red_partial);
-  hipFree(op->red_out);
+  (void)hipFree(op->cache64);
+  (void)hipFree(op->cache32);
+  (void)hipFree(op->part);
+  (void)hipFree(op->red_partial);
+  (void)hipFree(op->red_out);
   if (op->red_host)
-    hipHostFree(op->red_host);
-  hipFree(op->ptrs_dev);
+    (void)hipHostFree(op->red_host);
+  (void)hipFree(op->ptrs_dev);
   if (op->ptrs_host)
-    hipHostFree(op->ptrs_host);
-  hipFree(op->diag);
-  hipFree(op->inv_diag64);
-  hipFree(op->inv_diag32);
-  hipFree(op->blocks);
-  hipFree(op->inv_blocks64);
-  hipFree(op->inv_blocks32);
-  hipFree(op->d_singular);
+    (void)hipHostFree(op->ptrs_host);
+  (void)hipFree(op->diag);
+  (void)hipFree(op->inv_diag64);
+  (void)hipFree(op->inv_diag32);
+  (void)hipFree(op->blocks);
+  (void)hipFree(op->inv_blocks64);
+  (void)hipFree(op->inv_blocks32);
+  (void)hipFree(op->d_singular);
   for (int i = 0; i < 3; ++i)
     if (op->ev[i])
-      hipEventDestroy(op->ev[i]);
+      (void)hipEventDestroy(op->ev[i]);
   delete op;
   return HPS_OK;
 }

@@ -275,6 +275,7 @@ namespace hps
       dst[t.gnode(kp, nx, ny, k) * ostride + (int64_t)c * ocstride + ooffset] = v;
   }
 
+  // waves per SIMD the register allocation of q1_apply aims at
   // ------------------------------------------- vmult / residual kernel
   template <int DIM, int TX, int TY, int G, bool TENS, int OP, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
@@ -358,6 +359,7 @@ namespace hps
     store_plane(1);
     __syncthreads();
 
+    // uniform base of this tile's cache + a 32-bit per-lane offset
     const T *cache_tile = cache + (int64_t)blockIdx.x * kp.tz * NF * NT;
 
     for (int k = 0; k < t.tm_eff; ++k)
@@ -367,8 +369,8 @@ namespace hps
           load_plane(k + 2);
 
         // ---- cell layer k: interpolate, physics, integrate
-        T   val[NC], grad[NC][DIM];
         const T *pl = &ring[(k + t.bm) & 1][0][t.pn_lane * NC];
+        T        val[NC], grad[NC][DIM];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           interp_vg<DIM>(pl[c], t.corner, inv_h, val[c], grad[c]);
@@ -377,10 +379,10 @@ namespace hps
         if constexpr (OP == OP_VMULT)
           {
             T lin[NF];
-            const T *cl = cache_tile + (int64_t)k * NF * NT + tid;
+            const T *cl = cache_tile + k * (NF * NT);
 #pragma unroll
             for (int f = 0; f < NF; ++f)
-              lin[f] = cl[f * NT];
+              lin[f] = cl[f * NT + tid];
             tangent_qp<DIM, G, TENS>(kp.pp, lin, val, grad, vr, gr);
           }
         else
@@ -725,13 +727,17 @@ namespace hps
     return g > 0 && g < nc && (g % tsz) == 0;
   }
 
+  // one thread per (shared node, component); the three classes of shared
+  // nodes (on a march-direction tile boundary; on a y boundary only; on an x
+  // boundary only) are enumerated back to back, x fastest where possible
   template <typename T>
   __global__ void __launch_bounds__(256)
     q1_fixup(const FixParams fp, const T *__restrict__ part, T *__restrict__ dst)
   {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int     g[3];
-    // class M: planes on march boundaries (all x, y)
+    int64_t   idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c   = (int)(idx % fp.n_comp);
+    idx /= fp.n_comp;
+    int g[3];
     if (idx < fp.n_m)
       {
         g[0] = (int)(idx % fp.nn[0]);
@@ -750,10 +756,11 @@ namespace hps
       }
     else if ((idx -= fp.n_y) < fp.n_x)
       {
+        const int64_t per = fp.nb[0];
+        g[0] = (int)(idx % per + 1) * fp.tsz[0];
+        idx /= per;
         g[1] = (int)(idx % fp.nn[1]);
-        idx /= fp.nn[1];
-        g[2] = (int)(idx % fp.nn[2]);
-        g[0] = (int)(idx / fp.nn[2] + 1) * fp.tsz[0];
+        g[2] = (int)(idx / fp.nn[1]);
         if (is_bnd(g[2], fp.tsz[2], fp.nc[2]) || is_bnd(g[1], fp.tsz[1], fp.nc[1]))
           return;
       }
@@ -778,22 +785,20 @@ namespace hps
             ns[d]    = 1;
             tl[d][0] = min(g[d] / fp.tsz[d], fp.nt[d] - 1);
             lo[d][0] = g[d] - tl[d][0] * fp.tsz[d];
+            tl[d][1] = tl[d][0];
+            lo[d][1] = lo[d][0];
           }
       }
     const int64_t node = ((int64_t)g[2] * fp.nn[1] + g[1]) * fp.nn[0] + g[0];
-    for (int c = 0; c < fp.n_comp; ++c)
-      {
-        T s = T(0);
-        for (int a = 0; a < ns[2]; ++a)
-          for (int b = 0; b < ns[1]; ++b)
-            for (int e = 0; e < ns[0]; ++e)
-              {
-                const int64_t tile =
-                  ((int64_t)tl[2][a] * fp.nt[1] + tl[1][b]) * fp.nt[0] + tl[0][e];
-                const int pn = lo[0][e] + fp.px * lo[1][b];
-                s += part[((tile * (fp.tsz[2] + 1) + lo[2][a]) * fp.pn + pn) * fp.n_comp + c];
-              }
-        dst[node * fp.ostride + (int64_t)c * fp.ocstride + fp.ooffset] = s;
-      }
+    T             s    = T(0);
+    for (int a = 0; a < ns[2]; ++a)
+      for (int b = 0; b < ns[1]; ++b)
+        for (int e = 0; e < ns[0]; ++e)
+          {
+            const int64_t tile = ((int64_t)tl[2][a] * fp.nt[1] + tl[1][b]) * fp.nt[0] + tl[0][e];
+            const int     pn   = lo[0][e] + fp.px * lo[1][b];
+            s += part[((tile * (fp.tsz[2] + 1) + lo[2][a]) * fp.pn + pn) * fp.n_comp + c];
+          }
+    dst[node * fp.ostride + (int64_t)c * fp.ocstride + fp.ooffset] = s;
   }
 } // namespace hps

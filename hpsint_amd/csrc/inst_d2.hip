@@ -4,11 +4,14 @@
 namespace hps
 {
   const OpTable *q1_table_d3_scalar(int n_grains);
+  const OpTable *q1_table_d3_scalar_ty8(int n_grains);
   const OpTable *q1_table_d3_tensorial(int n_grains);
 
-  const OpTable *q1_table_d3(int n_grains, bool tensorial)
+  const OpTable *q1_table_d3(int n_grains, bool tensorial, int ty)
   {
-    return tensorial ? q1_table_d3_tensorial(n_grains) : q1_table_d3_scalar(n_grains);
+    if (tensorial)
+      return q1_table_d3_tensorial(n_grains);
+    return ty == 8 ? q1_table_d3_scalar_ty8(n_grains) : q1_table_d3_scalar(n_grains);
   }
 
   const OpTable *q1_table_d2(int n_grains, bool tensorial)
@@ -43,7 +46,7 @@ namespace hps
   template <typename T>
   void launch_fixup(const FixParams &fp, const T *part, T *dst, hipStream_t s)
   {
-    const int64_t n = fp.n_m + fp.n_y + fp.n_x;
+    const int64_t n = (fp.n_m + fp.n_y + fp.n_x) * fp.n_comp;
     if (n == 0)
       return;
     const int blocks = (int)((n + 255) / 256);

@@ -81,7 +81,8 @@ namespace hps
   constexpr int TX2 = 64, TY2 = 1;
 
   // nullptr when (n_grains, mobility) is not built for that dimension
-  const OpTable *q1_table_d3(int n_grains, bool tensorial);
+  // ty: tile extent along y (4, or 8 for scalar mobility)
+  const OpTable *q1_table_d3(int n_grains, bool tensorial, int ty);
   const OpTable *q1_table_d2(int n_grains, bool tensorial);
 
   template <typename T>

@@ -99,13 +99,15 @@ namespace hps
   // S/operator_sintering_generic.h:104-190 (advection disabled)
   // lin: cached fields; v[NC], g[NC][DIM]: increment; vr, gr: result
   // ------------------------------------------------------------------
-  template <int DIM, int G, bool TENS, typename T>
-  __device__ __forceinline__ void tangent_qp(const PhysParams &pp,
-                                             const T          *lin,
-                                             const T          *v,
-                                             const T (*g)[DIM],
-                                             T                *vr,
-                                             T (*gr)[DIM])
+  // VALS / GRADC select the value rows and the gradient row of c; the other
+  // gradient rows are always written
+  template <int DIM, int G, bool TENS, bool VALS, bool GRADC, typename T>
+  __device__ __forceinline__ void tangent_qp_rows(const PhysParams &pp,
+                                                  const T          *lin,
+                                                  const T          *v,
+                                                  const T (*g)[DIM],
+                                                  T                *vr,
+                                                  T (*gr)[DIM])
   {
     using CL        = CacheLayout<DIM, G, TENS>;
     const T  c      = lin[CL::c];
@@ -128,7 +130,10 @@ namespace hps
 
     // 1) c row
     vr[0] = v[0] * w0;
-    if constexpr (!TENS)
+    if constexpr (!GRADC)
+      {
+      }
+    else if constexpr (!TENS)
       {
         // MobilityScalar::apply_M_derivative, S/mobility.h:477-534
         T eta_sum = T(0.0), eta_ij_sum = T(0.0);
@@ -245,6 +250,18 @@ namespace hps
       }
 
     // 2) mu row
+    const T kc = -T(pp.kappa_c);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      gr[1][d] = kc * g[0][d];
+    const T Lkp = L * T(pp.kappa_p);
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        gr[i + 2][d] = Lkp * g[i + 2][d];
+    if constexpr (!VALS)
+      return;
     const T d2f_dc2 = T(2.0) * A * (c * c) + T(4.0) * A * c * (T(2.0) * c - T(2.0)) +
                       T(2.0) * A * ((-c + T(1.0)) * (-c + T(1.0))) + T(2.0) * B;
     T vmu = T(pp.w_alg) * v[1] - d2f_dc2 * v[0];
@@ -256,13 +273,8 @@ namespace hps
         vmu -= d2f_dcdeta[i] * v[i + 2];
       }
     vr[1] = vmu;
-    const T kc = -T(pp.kappa_c);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      gr[1][d] = kc * g[0][d];
 
     // 3) eta rows
-    const T Lkp = L * T(pp.kappa_p);
 #pragma unroll
     for (int i = 0; i < G; ++i)
       {
@@ -270,9 +282,6 @@ namespace hps
         const T d2f_de2  = B * (T(12.0) - T(12.0) * c + T(2.0) * e * (T(12.0) * c - T(24.0)) +
                                T(24.0) * (e * e) + T(12.0) * S2);
         vr[i + 2] = v[i + 2] * w0 + L * (d2f_dcdeta[i] * v[0] + d2f_de2 * v[i + 2]);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          gr[i + 2][d] = Lkp * g[i + 2][d];
       }
     // apply_d2f_detaidetaj, S/free_energy.h:234-252
     if constexpr (G > 1)
@@ -286,6 +295,56 @@ namespace hps
         for (int i = 0; i < G; ++i)
           vr[2 + i] += f * etas[i] * (temp - etas[i] * v[2 + i]);
       }
+  }
+
+  template <int DIM, int G, bool TENS, typename T>
+  __device__ __forceinline__ void tangent_qp(const PhysParams &pp, const T *lin, const T *v,
+                                             const T (*g)[DIM], T *vr, T (*gr)[DIM])
+  {
+    tangent_qp_rows<DIM, G, TENS, true, true>(pp, lin, v, g, vr, gr);
+  }
+
+  // ------------------------------------------------------------------
+  // The tangent split by output: the value rows need the increment's values
+  // only; the gradient row of c needs the gradients of c and mu; the other
+  // gradient rows are multiples of one input gradient each.  Kernels that
+  // stream one component at a time use these pieces; tangent_qp above is
+  // their composition.
+  // ------------------------------------------------------------------
+  template <int DIM, int G, bool TENS, typename T>
+  __device__ __forceinline__ void tangent_values(const PhysParams &pp, const T *lin,
+                                                 const T *v, T *vr)
+  {
+    T g[G + 2][DIM], gr[G + 2][DIM];
+#pragma unroll
+    for (int c = 0; c < G + 2; ++c)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        g[c][d] = T(0);
+    tangent_qp_rows<DIM, G, TENS, true, false>(pp, lin, v, g, vr, gr);
+  }
+
+  template <int DIM, int G, bool TENS, typename T>
+  __device__ __forceinline__ void tangent_grad_c(const PhysParams &pp, const T *lin,
+                                                 const T *v, const T *g0, const T *g1,
+                                                 T *out)
+  {
+    T g[G + 2][DIM], vr[G + 2], gr[G + 2][DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        g[0][d] = g0[d];
+        g[1][d] = g1[d];
+      }
+#pragma unroll
+    for (int c = 2; c < G + 2; ++c)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        g[c][d] = T(0);
+    tangent_qp_rows<DIM, G, TENS, false, true>(pp, lin, v, g, vr, gr);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      out[d] = gr[0][d];
   }
 
   // ------------------------------------------------------------------
@@ -413,5 +472,41 @@ namespace hps
         for (int d = 0; d < DIM; ++d)
           gr[2 + i][d] = Lkp * g[2 + i][d];
       }
+  }
+  // residual rows that need the state's values only (all rows' values)
+  template <int DIM, int G, bool TENS, int WTD, typename T>
+  __device__ __forceinline__ void residual_values(const PhysParams &pp, const T *v,
+                                                  const T *hist, T *vr)
+  {
+    T g[G + 2][DIM], gr[G + 2][DIM];
+#pragma unroll
+    for (int c = 0; c < G + 2; ++c)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        g[c][d] = T(0);
+    residual_qp<DIM, G, false, WTD>(pp, v, g, hist, vr, gr);
+  }
+
+  // gradient row of c of the residual for scalar mobility: M(c, eta) grad mu
+  // (MobilityScalar::apply_M, S/mobility.h:441-472)
+  template <int DIM, int G, typename T>
+  __device__ __forceinline__ void residual_grad_c_scalar(const PhysParams &pp, const T *v,
+                                                         const T *g1, T *out)
+  {
+    const T  c    = v[0];
+    const T *etas = v + 2;
+    T        etaijSum = T(0.0);
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+#pragma unroll
+      for (int j = 0; j < i; ++j)
+        etaijSum += etas[i] * etas[j];
+    const T phi = phi_clamped(c);
+    const T M   = T(pp.Mvol) * phi + T(pp.Mvap) * (T(1.0) - phi) +
+                T(pp.Msurf) * T(4.0) * c * c * (T(1.0) - c) * (T(1.0) - c) +
+                (T(2.0) * T(pp.Mgb)) * etaijSum;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      out[d] = M * g1[d];
   }
 } // namespace hps

@@ -234,3 +234,22 @@ def test_blas1_and_layout_conversions():
     back = op.initialize_dof_vector()
     _ffi.check(L.hps_from_blocks(h, P(back), arr))
     np.testing.assert_array_equal(host(back), x)
+
+
+@pytest.mark.parametrize("ty,tz", [(8, 16), (4, 5), (8, 64), (4, 128)])
+def test_vmult_and_residual_tile_geometries(monkeypatch, ty, tz):
+    """same results for every workgroup tile geometry"""
+    monkeypatch.setenv("HPSINT_AMD_TILE_Y", str(ty))
+    monkeypatch.setenv("HPSINT_AMD_TILE_M", str(tz))
+    op, p = make_pair(3, (19, 13, 37), 4, False, w0=100.0, w_hist=(-100.0,))
+    u = fields(p, 1)
+    src = np.random.default_rng(2).uniform(-1, 1, size=u.size)
+    ref = po.vmult(p, po.fill_qp(p, u), src)
+    op.fill_quadrature_point_values(dev(u))
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, dev(src))
+    assert rel_l2(host(dst), ref) <= TOL64
+    old = fields(p, 7)
+    ref = po.residual(p, 2, u, [old])
+    op.evaluate_nonlinear_residual(dst, dev(u), 2, old=[dev(old)])
+    assert rel_l2(host(dst), ref) <= TOL64

@@ -84,10 +84,10 @@ def emulate(dim, ncells, TX, TY, TZ):
                 continue
         else:
             i = idx - n_m - n_y
+            g[0] = (i % nb[0] + 1) * tsz[0]
+            i //= nb[0]
             g[1] = i % nn[1]
-            i //= nn[1]
-            g[2] = i % nn[2]
-            g[0] = (i // nn[2] + 1) * tsz[0]
+            g[2] = i // nn[1]
             if is_bnd(g[2], tsz[2], nc[2]) or is_bnd(g[1], tsz[1], nc[1]):
                 continue
         tl, lo = [], []
@@ -121,10 +121,13 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("tz", [16, 5, 64])
 @pytest.mark.parametrize("dim,ncells", CASES)
-def test_every_node_written_exactly_once(dim, ncells):
+def test_every_node_written_exactly_once(dim, ncells, tz):
     if dim == 3:
-        counts = emulate(3, ncells, 8, 4, 16)
+        for ty in (4, 8):
+            counts = emulate(3, ncells, 8, ty, tz)
+            assert all(c == 1 for c in counts), [i for i, c in enumerate(counts) if c != 1][:10]
     else:
-        counts = emulate(2, ncells, 64, 1, 16)
-    assert all(c == 1 for c in counts), [i for i, c in enumerate(counts) if c != 1][:10]
+        counts = emulate(2, ncells, 64, 1, tz)
+        assert all(c == 1 for c in counts), [i for i, c in enumerate(counts) if c != 1][:10]
