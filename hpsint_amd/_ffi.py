@@ -19,6 +19,8 @@ HPS_ERR_UNSUPPORTED = 3
 HPS_ERR_OUT_OF_MEMORY = 4
 HPS_ERR_HIP = 5
 HPS_ERR_NOT_INITIALIZED = 6
+HPS_ERR_NEWTON_NO_CONVERGENCE = 7
+HPS_ERR_LINEAR_NO_CONVERGENCE = 8
 
 HPS_MOBILITY_SCALAR = 0
 HPS_MOBILITY_TENSORIAL = 1
@@ -46,6 +48,35 @@ class HpsParams(ctypes.Structure):
     ]
 
 
+class HpsNewtonParams(ctypes.Structure):
+    _fields_ = [
+        ("nl_abs_tol", ctypes.c_double),
+        ("nl_rel_tol", ctypes.c_double),
+        ("nl_max_iter", ctypes.c_int),
+        ("l_abs_tol", ctypes.c_double),
+        ("l_rel_tol", ctypes.c_double),
+        ("l_max_iter", ctypes.c_int),
+        ("l_restart", ctypes.c_int),
+        ("precond", ctypes.c_int),
+        ("use_damping", ctypes.c_int),
+        ("threshold_newton_iter", ctypes.c_int),
+        ("threshold_linear_iter", ctypes.c_int),
+        ("inner_fp32", ctypes.c_int),
+    ]
+
+
+class HpsNewtonStats(ctypes.Structure):
+    _fields_ = [
+        ("newton_iterations", ctypes.c_int),
+        ("linear_iterations", ctypes.c_int),
+        ("residual_evaluations", ctypes.c_int),
+        ("precond_updates", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("res0", ctypes.c_double),
+        ("res", ctypes.c_double),
+    ]
+
+
 class HpsError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("hpsint_amd error %d: %s" % (code, msg))
@@ -54,6 +85,14 @@ class HpsError(RuntimeError):
 
 class InvalidNumberOfComponents(HpsError):
     """ExcInvalidNumberOfComponents (S/instantiation.h:61-68)"""
+
+
+class NewtonDidNotConverge(HpsError):
+    """ExcNewtonDidNotConverge (I/lac/solvers_nonlinear.h:183-207)"""
+
+
+class LinearSolverNoConvergence(HpsError):
+    """SolverControl::NoConvergence raised by GMRES"""
 
 
 _LIB = None
@@ -96,13 +135,17 @@ _SIGNATURES = {
     "hps_to_blocks": [vp, ctypes.POINTER(vp), vp],
     "hps_last_kernel_ms": [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "hps_enable_timing": [vp, cint],
+    "hps_newton_create": [vp, ctypes.POINTER(HpsNewtonParams), ctypes.POINTER(vp)],
+    "hps_newton_destroy": [vp],
+    "hps_newton_clear": [vp],
+    "hps_newton_solve": [vp, vp, ctypes.POINTER(vp), ctypes.POINTER(HpsNewtonStats)],
 }
 
 
 def header_symbols():
     """Names of the entry points declared in include/hpsint_amd.h."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(hps_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(hps_\w+)\s*\(", text, re.M)))
 
 
 def lib():
@@ -119,6 +162,8 @@ def lib():
             f.restype = cint
         L.hps_last_error.restype = ctypes.c_char_p
         L.hps_last_error.argtypes = []
+        L.hps_newton_default_params.restype = None
+        L.hps_newton_default_params.argtypes = [ctypes.POINTER(HpsNewtonParams)]
         _LIB = L
     return _LIB
 
@@ -128,4 +173,8 @@ def check(code):
         msg = lib().hps_last_error().decode()
         if code == HPS_ERR_INVALID_N_COMP:
             raise InvalidNumberOfComponents(code, msg)
+        if code == HPS_ERR_NEWTON_NO_CONVERGENCE:
+            raise NewtonDidNotConverge(code, msg)
+        if code == HPS_ERR_LINEAR_NO_CONVERGENCE:
+            raise LinearSolverNoConvergence(code, msg)
         raise HpsError(code, msg)

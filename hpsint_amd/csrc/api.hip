@@ -84,6 +84,11 @@ static int check_op(hps_operator op)
   return HPS_OK;
 }
 
+int hps_internal_fail(int code, const char *msg)
+{
+  return fail(code, msg);
+}
+
 extern "C" const char *hps_last_error(void)
 {
   return g_last_error.c_str();

@@ -1,0 +1,135 @@
+"""Newton-Krylov and time stepping over the C ABI.
+
+``NewtonSolver`` wraps hps_newton_* (the C++ restatement of
+DampedNewtonSolver + SolverGMRESWrapper, I/lac/solvers_nonlinear.h:248-357,
+I/lac/solvers_linear.h:58-117, with I/ = include/pf-applications/).
+``TimeStepper`` is the implicit time loop of the driver for fixed steps
+(S/driver.h:1932-2700 with S/ = applications/sintering/include/
+pf-applications/sintering/): set time weights, predict the initial guess
+(the predictor slot of S/driver.h:2002-2050), Newton solve, commit the old
+solutions (TimeIntegration::SolutionHistory).  On non-convergence the step
+is retried with dt/2 (process_failure, S/driver.h:2163-2200).
+"""
+import ctypes
+import time
+from dataclasses import dataclass
+
+import torch
+
+from . import _ffi
+from ._ffi import check, lib
+from .operator import SinteringOperatorGeneric, TimeIntegratorData
+
+
+class NewtonSolver:
+    PRECONDITIONERS = {"InverseDiagonalMatrix": _ffi.HPS_PRECON_POINT_JACOBI,
+                       "InverseComponentBlockDiagonalMatrix": _ffi.HPS_PRECON_NODE_BLOCK_JACOBI}
+
+    def __init__(self, op: SinteringOperatorGeneric, nl_rel_tol=1e-5, nl_abs_tol=1e-20,
+                 nl_max_iter=10, l_rel_tol=1e-2, l_abs_tol=1e-10, l_max_iter=1000,
+                 l_restart=30, preconditioner="InverseComponentBlockDiagonalMatrix",
+                 use_damping=True, threshold_newton_iter=100, threshold_linear_iter=20,
+                 inner_fp32=False):
+        self.op = op
+        p = _ffi.HpsNewtonParams()
+        lib().hps_newton_default_params(ctypes.byref(p))
+        p.nl_rel_tol, p.nl_abs_tol, p.nl_max_iter = nl_rel_tol, nl_abs_tol, nl_max_iter
+        p.l_rel_tol, p.l_abs_tol, p.l_max_iter = l_rel_tol, l_abs_tol, l_max_iter
+        p.l_restart = l_restart
+        p.precond = self.PRECONDITIONERS[preconditioner]
+        p.use_damping = int(use_damping)
+        p.threshold_newton_iter = threshold_newton_iter
+        p.threshold_linear_iter = threshold_linear_iter
+        p.inner_fp32 = int(inner_fp32)
+        self.params = p
+        h = ctypes.c_void_p()
+        check(lib().hps_newton_create(op.handle, ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _ffi._LIB is not None:
+            lib().hps_newton_destroy(h)
+            self._h = None
+
+    def clear(self):
+        check(lib().hps_newton_clear(self._h))
+
+    def solve(self, x, old):
+        """solves R(x) = 0 in place; returns the statistics"""
+        arr = (ctypes.c_void_p * max(1, len(old)))(*[o.data_ptr() for o in old])
+        st = _ffi.HpsNewtonStats()
+        rc = lib().hps_newton_solve(self._h, ctypes.c_void_p(x.data_ptr()), arr,
+                                    ctypes.byref(st))
+        self.last_stats = st
+        check(rc)
+        return st
+
+
+@dataclass
+class StepRecord:
+    t: float
+    dt: float
+    newton_iterations: int
+    linear_iterations: int
+    residual_evaluations: int
+    seconds: float
+    predictor_seconds: float = 0.0
+
+
+class TimeStepper:
+    """Fixed-step implicit BDF time loop with a pluggable predictor."""
+
+    def __init__(self, op: SinteringOperatorGeneric, solver: NewtonSolver,
+                 order=1, dt=1e-3, predictor=None, max_failures=5):
+        self.op = op
+        self.solver = solver
+        self.td = TimeIntegratorData(order=order, dt=dt)
+        self.dt = dt
+        self.order = order
+        self.predictor = predictor
+        self.max_failures = max_failures
+        self.history = []     # old solutions, most recent first
+        self.t = 0.0
+        self.records = []
+
+    def initialize(self, u0):
+        self.u = u0.clone()
+        self.history = [u0.clone() for _ in range(self.order)]
+
+    def make_step(self):
+        dt = self.dt
+        failures = 0
+        while True:
+            self.td.replace_dt(dt)
+            self.op.set_time_data(self.td)
+            eo = min(self.order, len(self.history))
+            old = self.history[:eo]
+            t0 = time.perf_counter()
+            x = self.u.clone()
+            tp = 0.0
+            if self.predictor is not None:
+                tp0 = time.perf_counter()
+                self.predictor(x, self.history, dt)
+                torch.cuda.synchronize()
+                tp = time.perf_counter() - tp0
+            try:
+                st = self.solver.solve(x, old)
+            except (_ffi.NewtonDidNotConverge, _ffi.LinearSolverNoConvergence):
+                failures += 1
+                if failures > self.max_failures:
+                    raise
+                dt *= 0.5
+                self.solver.clear()
+                continue
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
+            break
+        self.t += dt
+        self.history = [x.clone()] + self.history[:self.order - 1]
+        self.u = x
+        self.td.update_dt(dt)
+        rec = StepRecord(self.t, dt, st.newton_iterations, st.linear_iterations,
+                         st.residual_evaluations, sec, tp)
+        self.records.append(rec)
+        return rec

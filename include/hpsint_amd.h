@@ -40,7 +40,9 @@ enum
   HPS_ERR_UNSUPPORTED       = 3, /* dim / degree / mobility not built */
   HPS_ERR_OUT_OF_MEMORY     = 4,
   HPS_ERR_HIP               = 5,
-  HPS_ERR_NOT_INITIALIZED   = 6  /* vmult before set_linearization etc. */
+  HPS_ERR_NOT_INITIALIZED   = 6, /* vmult before set_linearization etc. */
+  HPS_ERR_NEWTON_NO_CONVERGENCE = 7, /* ExcNewtonDidNotConverge, I/lac/solvers_nonlinear.h:183-207 */
+  HPS_ERR_LINEAR_NO_CONVERGENCE = 8  /* SolverControl::NoConvergence (GMRES) */
 };
 
 enum
@@ -169,6 +171,51 @@ int hps_to_blocks(hps_operator op, double *const *blocks, const double *x);
  * (main kernel + shared-node fix-up); for measurement only */
 int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fixup);
 int hps_enable_timing(hps_operator op, int enable);
+
+/* ------------------------------------------------------------------
+ * Newton-Krylov driver (host logic over the device operator).
+ * hps_newton_solve restates DampedNewtonSolver::solve
+ * (I/lac/solvers_nonlinear.h:248-357) with the TimeMarchingImplicit wiring
+ * (I/time_integration/time_marching.h:129-241): residual, setup_jacobian =
+ * hps_set_linearization, setup_preconditioner = hps_precond_update (when
+ * history_newton_iterations % threshold_newton_iter == 0 or the last linear
+ * solve took more than threshold_linear_iter iterations), solve_with_jacobian
+ * = restarted GMRES with left preconditioning and classical Gram-Schmidt
+ * (SolverGMRESWrapper, I/lac/solvers_linear.h:58-117, ReductionControl
+ * semantics), damped update (tau = 0.1, at most 100 halvings).
+ * ------------------------------------------------------------------ */
+typedef struct hps_newton_s *hps_newton;
+
+typedef struct
+{
+  double nl_abs_tol, nl_rel_tol; /* NonLinearData, solvers_nonlinear_parameters.h:39-67 */
+  int    nl_max_iter;
+  double l_abs_tol, l_rel_tol;
+  int    l_max_iter;
+  int    l_restart;              /* GMRES basis size (<= 31) */
+  int    precond;                /* HPS_PRECON_* */
+  int    use_damping;
+  int    threshold_newton_iter;
+  int    threshold_linear_iter;
+  int    inner_fp32;             /* 1: FP32 vmult and preconditioner inside GMRES */
+} hps_newton_params;
+
+typedef struct
+{
+  int    newton_iterations, linear_iterations, residual_evaluations, precond_updates;
+  int    converged;
+  double res0, res;              /* l2 norms of the initial and final residual */
+} hps_newton_stats;
+
+void hps_newton_default_params(hps_newton_params *params);
+int  hps_newton_create(hps_operator op, const hps_newton_params *params, hps_newton *out);
+int  hps_newton_destroy(hps_newton solver);
+/* DampedNewtonSolver::clear: the next solve sets the preconditioner up again */
+int  hps_newton_clear(hps_newton solver);
+/* solves R(x) = 0 in place (x: device vector, initial guess on entry) with
+ * the operator's current time weights and the history vectors old[k] */
+int  hps_newton_solve(hps_newton solver, double *x, const double *const *old,
+                      hps_newton_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,82 @@
+"""Newton-Krylov on the GPU against the oracle's nonlinear equations.
+
+The GPU solution of an implicit BDF1 step must satisfy the oracle's residual
+to the Newton tolerance (size-independent check of the whole solve), with
+FP64 and with the FP32 inner Krylov operator; the time loop must advance
+the two-particle neck-growth problem (2D, Q1 160x80, A=16, B=1, kappa_c=1,
+kappa_p=0.5, Mvol=1e-2, Mvap=1e-10, Msurf=4, Mgb=0.4, L=1, dt=1e-3).
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,  # noqa: E402
+                        SinteringOperatorGeneric, packing)
+from hpsint_amd.solvers import NewtonSolver, TimeStepper  # noqa: E402
+
+
+def setup_two_particles(n_cells=(160, 80)):
+    pk = packing.two_particles_2d()
+    mesh = CartesianMesh(n_cells, pk.lower, pk.upper)
+    op = SinteringOperatorGeneric(mesh, 2, FreeEnergyParams(), MobilityParams())
+    u0 = packing.initial_fields(pk, n_cells)
+    return op, mesh, u0
+
+
+def oracle_params(mesh, dt, nc=4):
+    return po.make_params(mesh.dim, 1, mesh.n_cells, mesh.cell_size, nc, A=16.0, B=1.0,
+                          kappa_c=1.0, kappa_p=0.5, Mvol=1e-2, Mvap=1e-10, Msurf=4.0,
+                          Mgb=0.4, L=1.0, w0=1.0 / dt, w_hist=(-1.0 / dt,))
+
+
+@pytest.mark.parametrize("inner_fp32", [False, True])
+@pytest.mark.parametrize("precond", ["InverseComponentBlockDiagonalMatrix",
+                                     "InverseDiagonalMatrix"])
+def test_newton_step_satisfies_oracle_residual(inner_fp32, precond):
+    op, mesh, u0 = setup_two_particles()
+    dt = 1e-3
+    op.set_time_weights(1.0 / dt, (-1.0 / dt,))
+    solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2, preconditioner=precond,
+                          inner_fp32=inner_fp32)
+    old = torch.from_numpy(u0).cuda()
+    x = old.clone()
+    st = solver.solve(x, [old])
+    assert st.converged and st.newton_iterations >= 1
+    assert st.res <= 1e-8 * st.res0
+    p = oracle_params(mesh, dt)
+    xs = x.cpu().numpy()
+    r0 = np.linalg.norm(po.residual(p, 2, u0, [u0]))
+    r = np.linalg.norm(po.residual(p, 2, xs, [u0]))
+    # same tolerance as the solve, with room for the summation-order difference
+    assert r <= 2e-8 * r0, (r, r0)
+
+
+def test_time_loop_two_particles_10_steps():
+    op, mesh, u0 = setup_two_particles()
+    solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2)
+    ts = TimeStepper(op, solver, order=1, dt=1e-3)
+    ts.initialize(torch.from_numpy(u0).cuda())
+    for _ in range(9):
+        assert ts.make_step().newton_iterations >= 1
+    prev = ts.u.cpu().numpy()
+    ts.make_step()
+    assert abs(ts.t - 1e-2) < 1e-12
+    u = ts.u.cpu().numpy()
+    # mass of c is conserved by the Cahn-Hilliard equation (zero-flux walls):
+    # the c-rows of the mass matrix applied to c sum to the same value
+    p0 = po.make_params(2, 1, mesh.n_cells, mesh.cell_size, 4, A=0, B=0, kappa_c=0,
+                        kappa_p=0, Mvol=0, Mvap=0, Msurf=0, Mgb=0, L=0, w0=1.0)
+    m0 = po.residual(p0, 1, u0).reshape(-1, 4)[:, 0].sum()
+    m1 = po.residual(p0, 1, u).reshape(-1, 4)[:, 0].sum()
+    assert abs(m1 - m0) <= 1e-6 * abs(m0), (m0, m1)
+    # the last step satisfies the oracle's equations to the Newton tolerance
+    p = oracle_params(mesh, 1e-3)
+    r0 = np.linalg.norm(po.residual(p, 2, prev, [prev]))
+    r = np.linalg.norm(po.residual(p, 2, u, [prev]))
+    assert r <= 2e-8 * r0, (r, r0)
+    # the neck grows: the field moves
+    assert np.linalg.norm(u - u0) > 0
