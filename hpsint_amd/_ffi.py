@@ -139,6 +139,17 @@ _SIGNATURES = {
     "hps_newton_destroy": [vp],
     "hps_newton_clear": [vp],
     "hps_newton_solve": [vp, vp, ctypes.POINTER(vp), ctypes.POINTER(HpsNewtonStats)],
+    "hps_newton_solve_ref": [vp, vp, ctypes.POINTER(vp), dbl, ctypes.POINTER(HpsNewtonStats)],
+    "hps_mlp_gemm": [vp, cint, cint, cint, cint, cint, cint, vp, cint, vp, cint, vp, cint, vp,
+                     vp, cint],
+    "hps_mlp_sum_splits": [vp, i64, cint, vp, vp],
+    "hps_mlp_colsum": [vp, cint, cint, vp, vp],
+    "hps_mlp_loss": [vp, i64, vp, vp, vp, vp, cint, ctypes.POINTER(ctypes.c_float)],
+    "hps_mlp_adam": [vp, cint, cint, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
+                     ctypes.c_float, ctypes.c_float, cint, vp],
+    "hps_mlp_features": [vp, vp, vp, dbl, dbl, dbl, vp, vp],
+    "hps_mlp_predict": [vp, vp, vp, vp, vp, vp, vp],
+    "hps_mlp_targets": [vp, vp, vp, vp, vp, vp, vp],
 }
 
 

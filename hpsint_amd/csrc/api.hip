@@ -2,6 +2,7 @@
 // include/hpsint_amd.h for the contract of every entry point).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -9,6 +10,7 @@
 
 #include "../../include/hpsint_amd.h"
 #include "blas.hpp"
+#include "mlp.hpp"
 #include "ops.hpp"
 
 namespace
@@ -43,6 +45,9 @@ namespace
   // tile geometry: cells along y per workgroup (3D) and cells per tile along
   // the march direction; HPSINT_AMD_TILE_Y / HPSINT_AMD_TILE_M override the
   // defaults for measurements
+  // pinned host buffer of the reductions (and of the per-block loss partials)
+  constexpr int RED_HOST_DOUBLES = 4096;
+
   int env_int(const char *name, int dflt)
   {
     const char *v = std::getenv(name);
@@ -207,7 +212,7 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
     return cleanup_fail(e);
   if ((e = hipMalloc(&op->red_out, 64 * sizeof(double))) != hipSuccess)
     return cleanup_fail(e);
-  if ((e = hipHostMalloc(&op->red_host, 64 * sizeof(double))) != hipSuccess)
+  if ((e = hipHostMalloc(&op->red_host, RED_HOST_DOUBLES * sizeof(double))) != hipSuccess)
     return cleanup_fail(e);
   if ((e = hipMalloc(&op->ptrs_dev, 32 * sizeof(double *))) != hipSuccess)
     return cleanup_fail(e);
@@ -698,5 +703,141 @@ extern "C" int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fix
     *ms_main = a;
   if (ms_fixup)
     *ms_fixup = b;
+  return HPS_OK;
+}
+
+// ------------------------------------------------------------------
+// learned Newton predictor kernels (no counterpart in the reference; the
+// predictor slot is S/driver.h:2002-2050)
+// ------------------------------------------------------------------
+extern "C" int hps_mlp_gemm(hps_operator op, int trans_a, int trans_b, int epilogue, int M,
+                            int N, int K, const void *A, int lda, const void *B, int ldb,
+                            void *C, int ldc, const float *bias, const void *H, int n_split)
+{
+  if (int r = check_op(op))
+    return r;
+  if (M < 0 || N < 0 || K < 0 || n_split < 1 || !A || !B || !C)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "mlp_gemm: bad arguments");
+  if ((epilogue == hps::EPI_BIAS_TANH || epilogue == hps::EPI_BIAS_F32) && !bias)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "mlp_gemm: bias missing");
+  if (epilogue == hps::EPI_DTANH && !H)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "mlp_gemm: activation missing");
+  if (n_split > 1 && epilogue != hps::EPI_F32)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "mlp_gemm: split-K needs the FP32 epilogue");
+  const bool ok = (trans_a == 0 && trans_b == 1) || (trans_a == 0 && trans_b == 0 &&
+                   (epilogue == hps::EPI_DTANH || epilogue == hps::EPI_F32)) ||
+                  (trans_a == 1 && epilogue == hps::EPI_F32);
+  if (!ok)
+    return fail(HPS_ERR_UNSUPPORTED, "mlp_gemm: variant not built");
+  hps::mlp_gemm(op->stream, trans_a, trans_b, epilogue, M, N, K, (const uint16_t *)A, lda,
+                (const uint16_t *)B, ldb, C, ldc, bias, (const uint16_t *)H, n_split,
+                (int64_t)M * ldc);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_sum_splits(hps_operator op, int64_t n, int n_split, const float *part,
+                                  float *out)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::mlp_sum_splits(op->stream, n, n_split, part, out);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_colsum(hps_operator op, int M, int N, const void *X, float *out)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::mlp_colsum(op->stream, M, N, (const uint16_t *)X, out);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_loss(hps_operator op, int64_t n, const float *Y, const float *T,
+                            void *dY, float *partial, int n_blocks, float *loss)
+{
+  if (int r = check_op(op))
+    return r;
+  if (n_blocks < 1 || n_blocks > 2 * RED_HOST_DOUBLES)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "mlp_loss: n_blocks must be 1..8192");
+  hps::mlp_loss(op->stream, n, Y, T, (uint16_t *)dY, partial, n_blocks);
+  HPS_LAUNCHED();
+  if (loss)
+    {
+      float *h = (float *)op->red_host;
+      HPS_HIP(hipMemcpyAsync(h, partial, n_blocks * sizeof(float), hipMemcpyDeviceToHost,
+                             op->stream));
+      HPS_HIP(hipStreamSynchronize(op->stream));
+      double s = 0.0;
+      for (int i = 0; i < n_blocks; ++i)
+        s += h[i];
+      *loss = (float)s;
+    }
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_adam(hps_operator op, int n_out, int n_in, float *W, float *m1,
+                            float *m2, const float *grad, float lr, float beta1, float beta2,
+                            float eps, int step, void *W_bf16)
+{
+  if (int r = check_op(op))
+    return r;
+  if (step < 1)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "mlp_adam: step counts from 1");
+  const float c1 = 1.f - std::pow(beta1, (float)step);
+  const float c2 = 1.f - std::pow(beta2, (float)step);
+  hps::mlp_adam(op->stream, n_out, n_in, W, m1, m2, grad, lr, beta1, beta2, eps, c1, c2,
+                (uint16_t *)W_bf16, nullptr);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_features(hps_operator op, const double *u_prev, const double *u_cur,
+                                double dt, double dt_scale, double inc_scale, void *X, int *top2)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::FeatParams fp;
+  fp.n_nodes = op->n_nodes;
+  for (int d = 0; d < 3; ++d)
+    fp.nn[d] = 1;
+  fp.nn[0] = op->kp.nn[0];
+  if (op->dim == 3)
+    {
+      fp.nn[1] = op->kp.nn[1];
+      fp.nn[2] = op->kp.nn[2];
+    }
+  else
+    fp.nn[1] = op->kp.nn[2];
+  fp.n_comp    = op->nc;
+  fp.dt        = dt;
+  fp.dt_scale  = dt_scale;
+  fp.inc_scale = inc_scale;
+  hps::mlp_features(op->stream, fp, u_prev, u_cur, (uint16_t *)X, top2);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_predict(hps_operator op, const double *u_prev, const double *u_cur,
+                               const float *Y, const int *top2, const float *out_scale,
+                               double *u_pred)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::mlp_predict(op->stream, op->n_nodes, op->nc, u_prev, u_cur, Y, top2, out_scale, u_pred);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_mlp_targets(hps_operator op, const double *u_prev, const double *u_cur,
+                               const double *u_next, const int *top2, const float *out_scale,
+                               float *T)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::mlp_targets(op->stream, op->n_nodes, op->nc, u_prev, u_cur, u_next, top2, out_scale, T);
+  HPS_LAUNCHED();
   return HPS_OK;
 }

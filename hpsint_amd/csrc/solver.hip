@@ -291,6 +291,12 @@ extern "C" int hps_newton_clear(hps_newton nt)
 extern "C" int hps_newton_solve(hps_newton nt, double *x, const double *const *old,
                                 hps_newton_stats *st)
 {
+  return hps_newton_solve_ref(nt, x, old, -1.0, st);
+}
+
+extern "C" int hps_newton_solve_ref(hps_newton nt, double *x, const double *const *old,
+                                    double r0_ref, hps_newton_stats *st)
+{
   if (!nt || !x)
     return fail(HPS_ERR_INVALID_ARGUMENT, "null argument");
   hps_operator        op = nt->op;
@@ -307,8 +313,9 @@ extern "C" int hps_newton_solve(hps_newton nt, double *x, const double *const *o
   s.res0 = norm_r;
   if (!std::isfinite(norm_r))
     return fail(HPS_ERR_NEWTON_NO_CONVERGENCE, "non-finite initial residual");
-  // NewtonSolverSolverControl::check
-  const double r0   = norm_r;
+  // NewtonSolverSolverControl::check; r0_ref > 0 replaces the initial
+  // residual as the reference of the relative criterion
+  const double r0   = r0_ref > 0.0 ? r0_ref : norm_r;
   auto         keep = [&](int it, double v) {
     return v > p.nl_abs_tol && v / r0 > p.nl_rel_tol && it < p.nl_max_iter;
   };

@@ -251,6 +251,12 @@ class SinteringOperatorGeneric:
         check(lib().hps_precond_get_blocks(self._h, _ptr(out)))
         return out.view(self._n_nodes, nc, nc)
 
+    def norm(self, x):
+        """l2 norm of a device vector (DynamicBlockVector::l2_norm)"""
+        r = ctypes.c_double()
+        check(lib().hps_norm2(self._h, x.numel(), _ptr(x), ctypes.byref(r)))
+        return r.value
+
     def enable_timing(self, on=True):
         check(lib().hps_enable_timing(self._h, int(on)))
 

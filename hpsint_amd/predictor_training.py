@@ -1,0 +1,94 @@
+"""Training data and fitting of the learned Newton predictor.
+
+Trajectories come from the plain solver (Newton from u_n at the same
+tolerance).  For every consecutive triple (u_{n-1}, u_n, u_{n+1}) the
+library computes the stencil features of (u_{n-1}, u_n) and the targets
+(u_{n+1} - (2 u_n - u_{n-1})) / out_scale on the 4 outputs; a subset of
+nodes (the most active half by target magnitude plus a uniform sample) forms
+the training set.  Fitting is Adam on mini-batches, every product on the
+library's MFMA kernels.  torch only holds memory and selects rows.
+"""
+import math
+
+import torch
+
+from .predictor import MLP, N_IN, N_OUT, LearnedPredictor
+from .solvers import NewtonSolver, TimeStepper
+
+
+def collect_trajectory(op, u0, n_steps, dt, nl_rel_tol=1e-8, l_rel_tol=1e-2,
+                       preconditioner="InverseComponentBlockDiagonalMatrix", records=None):
+    solver = NewtonSolver(op, nl_rel_tol=nl_rel_tol, l_rel_tol=l_rel_tol,
+                          preconditioner=preconditioner)
+    ts = TimeStepper(op, solver, order=1, dt=dt)
+    ts.initialize(u0)
+    traj = [u0.clone()]
+    for _ in range(n_steps):
+        rec = ts.make_step()
+        if records is not None:
+            records.append(rec)
+        traj.append(ts.u.clone())
+    return traj
+
+
+def _rms(x):
+    return float(torch.sqrt(torch.mean(x.double() ** 2)))
+
+
+def scales(op, traj, dt):
+    """inc_scale, dt_scale, out_scale from a trajectory"""
+    nc = op.n_components()
+    inc = [traj[i] - traj[i - 1] for i in range(1, len(traj))]
+    inc_rms = math.sqrt(sum(_rms(d) ** 2 for d in inc) / len(inc))
+    sec = [(traj[i + 1] - 2 * traj[i] + traj[i - 1]).view(-1, nc) for i in range(1, len(traj) - 1)]
+    c = math.sqrt(sum(_rms(s[:, 0]) ** 2 for s in sec) / len(sec))
+    mu = math.sqrt(sum(_rms(s[:, 1]) ** 2 for s in sec) / len(sec))
+    eta = math.sqrt(sum(_rms(s[:, 2:]) ** 2 for s in sec) / len(sec))
+    out = [max(v, 1e-30) for v in (c, mu, eta, eta)]
+    return 1.0 / max(inc_rms, 1e-30), 1.0 / dt, out
+
+
+def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
+    op = pred.op
+    n = op.n_nodes
+    dev = pred.X.device
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    Xs, Ts = [], []
+    T = torch.empty(n, N_OUT, dtype=torch.float32, device=dev)
+    for i in range(1, len(traj) - 1):
+        X, _ = pred.features(traj[i - 1], traj[i], dt)
+        pred.targets(traj[i - 1], traj[i], traj[i + 1], T)
+        k = min(rows_per_step // 2, n)
+        act = T.abs().sum(dim=1)
+        top = torch.topk(act, k).indices
+        rnd = torch.randint(0, n, (rows_per_step - k,), generator=g).to(dev)
+        idx = torch.cat([top, rnd])
+        Xs.append(X.index_select(0, idx))
+        Ts.append(T.index_select(0, idx))
+    return torch.cat(Xs), torch.cat(Ts)
+
+
+def fit_predictor(op, traj, dt, iterations=1000, rows_per_step=32768, batch=16384,
+                  lr=1e-3, seed=0, log=None):
+    inc_scale, dt_scale, out_scale = scales(op, traj, dt)
+    mlp = MLP(op, seed=seed)
+    pred = LearnedPredictor(op, mlp, dt_scale=dt_scale, inc_scale=inc_scale,
+                            out_scale=torch.tensor(out_scale, dtype=torch.float32,
+                                                   device=mlp.W[0].device))
+    X, T = build_dataset(pred, traj, dt, rows_per_step, seed)
+    m = X.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(seed + 1)
+    b = min(batch, m)
+    Xb = torch.empty(b, N_IN, dtype=torch.bfloat16, device=X.device)
+    Tb = torch.empty(b, N_OUT, dtype=torch.float32, device=X.device)
+    losses = []
+    for it in range(iterations):
+        idx = torch.randint(0, m, (b,), generator=g).to(X.device)
+        torch.index_select(X, 0, idx, out=Xb)
+        torch.index_select(T, 0, idx, out=Tb)
+        lr_it = lr * 0.5 * (1.0 + math.cos(math.pi * it / iterations))
+        losses.append(mlp.train_step(Xb, Tb, b, lr_it))
+        if log is not None and (it % 100 == 0 or it == iterations - 1):
+            log(it, losses[-1])
+    pred.losses = losses
+    return pred

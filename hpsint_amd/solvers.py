@@ -55,12 +55,13 @@ class NewtonSolver:
     def clear(self):
         check(lib().hps_newton_clear(self._h))
 
-    def solve(self, x, old):
-        """solves R(x) = 0 in place; returns the statistics"""
+    def solve(self, x, old, r0_ref=-1.0):
+        """solves R(x) = 0 in place; returns the statistics.  r0_ref > 0 is
+        the reference norm of the relative criterion (default: |R(x_0)|)"""
         arr = (ctypes.c_void_p * max(1, len(old)))(*[o.data_ptr() for o in old])
         st = _ffi.HpsNewtonStats()
-        rc = lib().hps_newton_solve(self._h, ctypes.c_void_p(x.data_ptr()), arr,
-                                    ctypes.byref(st))
+        rc = lib().hps_newton_solve_ref(self._h, ctypes.c_void_p(x.data_ptr()), arr,
+                                        float(r0_ref), ctypes.byref(st))
         self.last_stats = st
         check(rc)
         return st
@@ -92,10 +93,11 @@ class TimeStepper:
         self.history = []     # old solutions, most recent first
         self.t = 0.0
         self.records = []
+        self.keep_guess = False
 
     def initialize(self, u0):
         self.u = u0.clone()
-        self.history = [u0.clone() for _ in range(self.order)]
+        self.history = [u0.clone()]   # committed solutions, most recent first
 
     def make_step(self):
         dt = self.dt
@@ -108,13 +110,21 @@ class TimeStepper:
             t0 = time.perf_counter()
             x = self.u.clone()
             tp = 0.0
+            r0_ref = -1.0
             if self.predictor is not None:
                 tp0 = time.perf_counter()
+                # the plain method's initial residual |R(u_n)| stays the
+                # reference of the relative Newton criterion
+                r = self._scratch if getattr(self, "_scratch", None) is not None else x.clone()
+                self._scratch = r
+                self.op.evaluate_nonlinear_residual(r, x, 2, old=old)
+                r0_ref = self.op.norm(r)
                 self.predictor(x, self.history, dt)
                 torch.cuda.synchronize()
                 tp = time.perf_counter() - tp0
+            self.last_guess = x.clone() if self.keep_guess else None
             try:
-                st = self.solver.solve(x, old)
+                st = self.solver.solve(x, old, r0_ref)
             except (_ffi.NewtonDidNotConverge, _ffi.LinearSolverNoConvergence):
                 failures += 1
                 if failures > self.max_failures:
@@ -126,7 +136,7 @@ class TimeStepper:
             sec = time.perf_counter() - t0
             break
         self.t += dt
-        self.history = [x.clone()] + self.history[:self.order - 1]
+        self.history = [x.clone()] + self.history[:max(self.order, 2) - 1]
         self.u = x
         self.td.update_dt(dt)
         rec = StepRecord(self.t, dt, st.newton_iterations, st.linear_iterations,

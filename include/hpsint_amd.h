@@ -216,6 +216,56 @@ int  hps_newton_clear(hps_newton solver);
  * the operator's current time weights and the history vectors old[k] */
 int  hps_newton_solve(hps_newton solver, double *x, const double *const *old,
                       hps_newton_stats *stats);
+/* as hps_newton_solve, but the relative criterion |R(x_k)| <= nl_rel_tol *
+ * r0_ref uses the given reference norm (e.g. |R(u_n)|, the initial residual
+ * of the plain method) instead of the initial residual of this solve, so
+ * that solves from different initial guesses stop at the same accuracy
+ * (r0_ref <= 0: identical to hps_newton_solve).  No counterpart in the
+ * reference; used with the learned predictor. */
+int  hps_newton_solve_ref(hps_newton solver, double *x, const double *const *old,
+                          double r0_ref, hps_newton_stats *stats);
+
+/* ------------------------------------------------------------------
+ * Learned Newton predictor (no counterpart in the reference: it fills the
+ * driver's predictor slot, S/driver.h:2002-2050, where the reference
+ * offers Euler/Midpoint/Linear extrapolation).  BF16 storage, FP32
+ * accumulation on MFMA.  Matrices are row-major; bf16 arrays are passed as
+ * void pointers to 16-bit storage.  All calls use the operator's stream.
+ * ------------------------------------------------------------------ */
+enum
+{
+  HPS_MLP_EPI_F32       = 0, /* C = op(A) op(B), FP32 (split-K partials at C + z*M*ldc) */
+  HPS_MLP_EPI_BIAS_TANH = 1, /* C = bf16(tanh(op(A) op(B) + bias)) */
+  HPS_MLP_EPI_BIAS_F32  = 2, /* C = op(A) op(B) + bias, FP32 */
+  HPS_MLP_EPI_DTANH     = 3  /* C = bf16(op(A) op(B) * (1 - H^2)) */
+};
+/* op(A)(m,k) = trans_a ? A[k*lda+m] : A[m*lda+k];
+ * op(B)(k,n) = trans_b ? B[n*ldb+k] : B[k*ldb+n];  M x N result */
+int hps_mlp_gemm(hps_operator op, int trans_a, int trans_b, int epilogue, int M, int N, int K,
+                 const void *A, int lda, const void *B, int ldb, void *C, int ldc,
+                 const float *bias, const void *H, int n_split);
+int hps_mlp_sum_splits(hps_operator op, int64_t n, int n_split, const float *part,
+                       float *out);
+int hps_mlp_colsum(hps_operator op, int M, int N, const void *X, float *out);
+/* MSE over n entries; dY = 2 (Y - T) / n (bf16); loss summed on the host if
+ * loss != NULL */
+int hps_mlp_loss(hps_operator op, int64_t n, const float *Y, const float *T, void *dY,
+                 float *partial, int n_blocks, float *loss);
+/* Adam step (step counts from 1) on FP32 master weights, writes bf16 copy */
+int hps_mlp_adam(hps_operator op, int n_out, int n_in, float *W, float *m1, float *m2,
+                 const float *grad, float lr, float beta1, float beta2, float eps, int step,
+                 void *W_bf16);
+/* per node: 272 bf16 stencil features and the node's top-2 order
+ * parameters at t_n (top2: 2 ints per node) */
+int hps_mlp_features(hps_operator op, const double *u_prev, const double *u_cur, double dt,
+                     double dt_scale, double inc_scale, void *X, int *top2);
+/* u_pred = 2 u_cur - u_prev, plus out_scale[k] * Y[node*4+k] on
+ * (c, mu, eta_top1, eta_top2) when Y != NULL */
+int hps_mlp_predict(hps_operator op, const double *u_prev, const double *u_cur,
+                    const float *Y, const int *top2, const float *out_scale, double *u_pred);
+/* training targets: (u_next - (2 u_cur - u_prev)) / out_scale on the 4 outputs */
+int hps_mlp_targets(hps_operator op, const double *u_prev, const double *u_cur,
+                    const double *u_next, const int *top2, const float *out_scale, float *T);
 
 #ifdef __cplusplus
 }
