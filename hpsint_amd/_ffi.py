@@ -130,6 +130,11 @@ _SIGNATURES = {
     "hps_multi_dot": [vp, i64, cint, vp, vp, pd],
     "hps_multi_axpy_norm": [vp, i64, cint, vp, pd, vp, pd],
     "hps_convert_f64_to_f32": [vp, i64, vp, vp],
+    "hps_norm2_f32": [vp, i64, vp, pd],
+    "hps_multi_dot_f32": [vp, i64, cint, vp, vp, pd],
+    "hps_multi_axpy_norm_f32": [vp, i64, cint, vp, pd, vp, pd],
+    "hps_scale_f32": [vp, i64, dbl, vp],
+    "hps_axpy_multi_f32": [vp, i64, cint, pd, vp, vp],
     "hps_convert_f32_to_f64": [vp, i64, vp, vp],
     "hps_from_blocks": [vp, vp, ctypes.POINTER(vp)],
     "hps_to_blocks": [vp, ctypes.POINTER(vp), vp],

@@ -841,3 +841,66 @@ extern "C" int hps_mlp_targets(hps_operator op, const double *u_prev, const doub
   HPS_LAUNCHED();
   return HPS_OK;
 }
+
+// ---- FP32-storage vectors with FP64 accumulation (mixed-precision Krylov)
+extern "C" int hps_norm2_f32(hps_operator op, int64_t n, const float *x, double *result)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_norm2_f32(op->stream, n, x, op->red_partial, op->red_out);
+  return fetch(op, 1, result);
+}
+
+extern "C" int hps_multi_dot_f32(hps_operator op, int64_t n, int k, const float *V,
+                                 const float *w, double *h)
+{
+  if (int r = check_op(op))
+    return r;
+  if (k < 1 || k > hps::MAXK)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "multi_dot_f32: k must be 1..32");
+  hps::blas_multi_dot_f32(op->stream, n, k, V, w, op->red_partial, op->red_out);
+  return fetch(op, k, h);
+}
+
+extern "C" int hps_multi_axpy_norm_f32(hps_operator op, int64_t n, int k, const float *V,
+                                       const double *h, float *w, double *norm)
+{
+  if (int r = check_op(op))
+    return r;
+  if (k < 1 || k > hps::MAXK)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "multi_axpy_norm_f32: k must be 1..32");
+  for (int j = 0; j < k; ++j)
+    op->red_host[32 + j] = h[j];
+  HPS_HIP(hipMemcpyAsync(op->red_out + 32, op->red_host + 32, k * sizeof(double),
+                         hipMemcpyHostToDevice, op->stream));
+  hps::blas_multi_axpy_norm_f32(op->stream, n, k, V, op->red_out + 32, w, op->red_partial,
+                                op->red_out);
+  return fetch(op, 1, norm);
+}
+
+extern "C" int hps_scale_f32(hps_operator op, int64_t n, double a, float *x)
+{
+  if (int r = check_op(op))
+    return r;
+  hps::blas_scale_f32(op->stream, n, a, x);
+  HPS_LAUNCHED();
+  return HPS_OK;
+}
+
+extern "C" int hps_axpy_multi_f32(hps_operator op, int64_t n, int k, const double *a,
+                                  const float *X, double *y)
+{
+  if (int r = check_op(op))
+    return r;
+  if (k < 1 || k > hps::MAXK)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "axpy_multi_f32: k must be 1..32");
+  for (int j = 0; j < k; ++j)
+    op->red_host[32 + j] = a[j];
+  HPS_HIP(hipMemcpyAsync(op->red_out + 32, op->red_host + 32, k * sizeof(double),
+                         hipMemcpyHostToDevice, op->stream));
+  hps::blas_axpy_multi_f32_f64(op->stream, n, k, op->red_out + 32, X, y);
+  HPS_LAUNCHED();
+  // the coefficients travel through the pinned buffer: wait before reuse
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  return HPS_OK;
+}

@@ -482,4 +482,137 @@ namespace hps
   {
     k_to_blocks<<<grid_for(n_nodes * nc, 256 * 4), 256, 0, s>>>(n_nodes, nc, blocks_dev, x);
   }
+  // ---- FP32 storage, FP64 accumulation (mixed-precision Krylov basis)
+  template <int KK>
+  __global__ void __launch_bounds__(RB)
+    k_multi_dot_f32(int64_t n, int k, const float *__restrict__ V, const float *__restrict__ w,
+                    double *__restrict__ partial)
+  {
+    __shared__ double sh[RB / 64];
+    double            acc[KK];
+#pragma unroll
+    for (int j = 0; j < KK; ++j)
+      acc[j] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * RB)
+      {
+        const double wi = w[i];
+#pragma unroll
+        for (int j = 0; j < KK; ++j)
+          if (j < k)
+            acc[j] += (double)V[(int64_t)j * n + i] * wi;
+      }
+#pragma unroll
+    for (int j = 0; j < KK; ++j)
+      if (j < k)
+        {
+          const double s = block_reduce(acc[j], sh);
+          if (threadIdx.x == 0)
+            partial[(int64_t)blockIdx.x * k + j] = s;
+        }
+  }
+
+  template <int KK>
+  __global__ void __launch_bounds__(RB)
+    k_multi_axpy_norm_f32(int64_t n, int k, const float *__restrict__ V,
+                          const double *__restrict__ h, float *__restrict__ w,
+                          double *__restrict__ partial)
+  {
+    __shared__ double sh[RB / 64];
+    double            hk[KK];
+#pragma unroll
+    for (int j = 0; j < KK; ++j)
+      hk[j] = j < k ? h[j] : 0.0;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * RB)
+      {
+        double wi = w[i];
+#pragma unroll
+        for (int j = 0; j < KK; ++j)
+          if (j < k)
+            wi -= hk[j] * (double)V[(int64_t)j * n + i];
+        const float wf = (float)wi;
+        w[i]           = wf;
+        s += (double)wf * (double)wf;
+      }
+    s = block_reduce(s, sh);
+    if (threadIdx.x == 0)
+      partial[blockIdx.x] = s;
+  }
+
+  __global__ void __launch_bounds__(RB)
+    k_norm_f32(int64_t n, const float *__restrict__ x, double *__restrict__ partial)
+  {
+    __shared__ double sh[RB / 64];
+    double            s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * RB)
+      s += (double)x[i] * (double)x[i];
+    s = block_reduce(s, sh);
+    if (threadIdx.x == 0)
+      partial[blockIdx.x] = s;
+  }
+
+  __global__ void k_scale_f32(int64_t n, double a, float *__restrict__ x)
+  {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      x[i] = (float)(a * (double)x[i]);
+  }
+
+  // y (FP64) += sum_j a_j x_j (FP32), j < k
+  __global__ void k_axpy_multi_f32_f64(int64_t n, int k, const double *__restrict__ a,
+                                       const float *__restrict__ X, double *__restrict__ y)
+  {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      {
+        double s = y[i];
+        for (int j = 0; j < k; ++j)
+          s += a[j] * (double)X[(int64_t)j * n + i];
+        y[i] = s;
+      }
+  }
+
+  void blas_multi_dot_f32(hipStream_t s, int64_t n, int k, const float *V, const float *w,
+                          double *partial, double *out)
+  {
+    if (k <= 8)
+      k_multi_dot_f32<8><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
+    else if (k <= 16)
+      k_multi_dot_f32<16><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
+    else
+      k_multi_dot_f32<MAXK><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
+    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, k, partial, out, 0);
+  }
+
+  void blas_multi_axpy_norm_f32(hipStream_t s, int64_t n, int k, const float *V,
+                                const double *h, float *w, double *partial, double *out)
+  {
+    if (k <= 8)
+      k_multi_axpy_norm_f32<8><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
+    else if (k <= 16)
+      k_multi_axpy_norm_f32<16><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
+    else
+      k_multi_axpy_norm_f32<MAXK><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
+    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+  }
+
+  void blas_norm2_f32(hipStream_t s, int64_t n, const float *x, double *partial, double *out)
+  {
+    k_norm_f32<<<RED_BLOCKS, RB, 0, s>>>(n, x, partial);
+    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+  }
+
+  void blas_scale_f32(hipStream_t s, int64_t n, double a, float *x)
+  {
+    k_scale_f32<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x);
+  }
+
+  void blas_axpy_multi_f32_f64(hipStream_t s, int64_t n, int k, const double *a, const float *X,
+                               double *y)
+  {
+    k_axpy_multi_f32_f64<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, k, a, X, y);
+  }
 } // namespace hps

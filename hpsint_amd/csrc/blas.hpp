@@ -36,4 +36,12 @@ namespace hps
                      float *dst);
   void from_blocks(hipStream_t, int64_t n_nodes, int nc, double *x, const double *const *blocks_dev);
   void to_blocks(hipStream_t, int64_t n_nodes, int nc, double *const *blocks_dev, const double *x);
+  void blas_multi_dot_f32(hipStream_t, int64_t n, int k, const float *V, const float *w,
+                          double *partial, double *out);
+  void blas_multi_axpy_norm_f32(hipStream_t, int64_t n, int k, const float *V, const double *h,
+                                float *w, double *partial, double *out);
+  void blas_norm2_f32(hipStream_t, int64_t n, const float *x, double *partial, double *out);
+  void blas_scale_f32(hipStream_t, int64_t n, double a, float *x);
+  void blas_axpy_multi_f32_f64(hipStream_t, int64_t n, int k, const double *a, const float *X,
+                               double *y);
 } // namespace hps

@@ -27,7 +27,7 @@ struct hps_newton_s
   hps_newton_params prm;
   int64_t           n;
   double           *r, *inc, *tmp, *w, *V, *z64;
-  float            *x32, *y32, *z32;
+  float            *x32, *y32, *z32, *V32;
   // DampedNewtonSolver history (persists across solves)
   unsigned          history_newton_iterations;
   unsigned          history_linear_iterations_last;
@@ -210,6 +210,138 @@ namespace
     *its = it;
     return HPS_OK;
   }
+  // Givens-rotation bookkeeping shared by both precisions
+  struct Arnoldi
+  {
+    int                 m;
+    std::vector<double> H, cs, sn, g, y, h;
+    explicit Arnoldi(int m_)
+      : m(m_), H((m_ + 1) * m_), cs(m_), sn(m_), g(m_ + 1), y(m_), h(m_ + 1)
+    {}
+    // rotates column j (h[0..j+1]) and returns the new residual estimate
+    double add_column(int j)
+    {
+      for (int i = 0; i < j; ++i)
+        {
+          const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+          h[i + 1]       = -sn[i] * h[i] + cs[i] * h[i + 1];
+          h[i]           = t;
+        }
+      const double den = std::hypot(h[j], h[j + 1]);
+      cs[j]            = den > 0.0 ? h[j] / den : 1.0;
+      sn[j]            = den > 0.0 ? h[j + 1] / den : 0.0;
+      h[j]             = den;
+      h[j + 1]         = 0.0;
+      g[j + 1]         = -sn[j] * g[j];
+      g[j]             = cs[j] * g[j];
+      for (int i = 0; i <= j; ++i)
+        H[i * m + j] = h[i];
+      return std::fabs(g[j + 1]);
+    }
+    void solve(int j)
+    {
+      for (int i = j - 1; i >= 0; --i)
+        {
+          double s = g[i];
+          for (int k = i + 1; k < j; ++k)
+            s -= H[i * m + k] * y[k];
+          y[i] = s / H[i * m + i];
+        }
+    }
+  };
+
+  // GMRES with the Krylov basis stored in FP32: FP32 tangent and
+  // preconditioner, Gram-Schmidt coefficients and norms accumulated in FP64,
+  // solution accumulated in FP64.  Same iteration and stopping rules as gmres.
+  int gmres_mixed(hps_newton nt, const double *b, double *x, int *its)
+  {
+    hps_operator op = nt->op;
+    const int64_t n = nt->n;
+    const int     m = nt->prm.l_restart;
+    float        *V = nt->V32;
+    Arnoldi       a(m);
+    HPS_TRY(hps_fill(op, n, 0.0, x));
+    HPS_TRY(hps_convert_f64_to_f32(op, n, nt->x32, b));
+    HPS_TRY(hps_precond_apply_f32(op, V, nt->x32));
+    double beta = 0.0;
+    HPS_TRY(hps_norm2_f32(op, n, V, &beta));
+    const double target = std::max(nt->prm.l_abs_tol, nt->prm.l_rel_tol * beta);
+    int          it     = 0;
+    if (beta <= target)
+      {
+        *its = 0;
+        return HPS_OK;
+      }
+    bool first = true;
+    while (true)
+      {
+        if (!first)
+          {
+            HPS_TRY(hps_convert_f64_to_f32(op, n, nt->x32, x));
+            HPS_TRY(hps_vmult_f32(op, nt->y32, nt->x32));
+            HPS_TRY(hps_convert_f32_to_f64(op, n, nt->z64, nt->y32));
+            HPS_TRY(hps_sadd(op, n, -1.0, 1.0, b, nt->z64));
+            HPS_TRY(hps_convert_f64_to_f32(op, n, nt->x32, nt->z64));
+            HPS_TRY(hps_precond_apply_f32(op, V, nt->x32));
+            HPS_TRY(hps_norm2_f32(op, n, V, &beta));
+            if (beta <= target)
+              break;
+          }
+        first = false;
+        HPS_TRY(hps_scale_f32(op, n, 1.0 / beta, V));
+        std::fill(a.g.begin(), a.g.end(), 0.0);
+        a.g[0] = beta;
+        int  j = 0;
+        bool converged = false;
+        for (; j < m; ++j)
+          {
+            float *vj1 = V + (int64_t)(j + 1) * n;
+            HPS_TRY(hps_vmult_f32(op, nt->y32, V + (int64_t)j * n));
+            HPS_TRY(hps_precond_apply_f32(op, vj1, nt->y32));
+            HPS_TRY(hps_multi_dot_f32(op, n, j + 1, V, vj1, a.h.data()));
+            double wnorm = 0.0;
+            HPS_TRY(hps_multi_axpy_norm_f32(op, n, j + 1, V, a.h.data(), vj1, &wnorm));
+            double hh = 0.0;
+            for (int i = 0; i <= j; ++i)
+              hh += a.h[i] * a.h[i];
+            if (wnorm < 0.5 * std::sqrt(wnorm * wnorm + hh))
+              {
+                std::vector<double> h2(j + 1);
+                HPS_TRY(hps_multi_dot_f32(op, n, j + 1, V, vj1, h2.data()));
+                HPS_TRY(hps_multi_axpy_norm_f32(op, n, j + 1, V, h2.data(), vj1, &wnorm));
+                for (int i = 0; i <= j; ++i)
+                  a.h[i] += h2[i];
+              }
+            a.h[j + 1]       = wnorm;
+            const double res = a.add_column(j);
+            ++it;
+            if (res <= target || wnorm == 0.0)
+              {
+                converged = true;
+                ++j;
+                break;
+              }
+            if (it >= nt->prm.l_max_iter)
+              {
+                ++j;
+                break;
+              }
+            HPS_TRY(hps_scale_f32(op, n, 1.0 / wnorm, vj1));
+          }
+        a.solve(j);
+        HPS_TRY(hps_axpy_multi_f32(op, n, j, a.y.data(), V, x));
+        if (converged)
+          break;
+        if (it >= nt->prm.l_max_iter)
+          {
+            *its = it;
+            return fail(HPS_ERR_LINEAR_NO_CONVERGENCE,
+                        "GMRES did not converge in " + std::to_string(it) + " iterations");
+          }
+      }
+    *its = it;
+    return HPS_OK;
+  }
 } // namespace
 
 extern "C" void hps_newton_default_params(hps_newton_params *p)
@@ -250,13 +382,15 @@ extern "C" int hps_newton_create(hps_operator op, const hps_newton_params *prm,
   e = e ? e : hipMalloc(&nt->inc, b64);
   e = e ? e : hipMalloc(&nt->tmp, b64);
   e = e ? e : hipMalloc(&nt->z64, b64);
-  e = e ? e : hipMalloc(&nt->V, b64 * (prm->l_restart + 1));
   if (prm->inner_fp32)
     {
+      e = e ? e : hipMalloc(&nt->V32, b32 * (prm->l_restart + 1));
       e = e ? e : hipMalloc(&nt->x32, b32);
       e = e ? e : hipMalloc(&nt->y32, b32);
       e = e ? e : hipMalloc(&nt->z32, b32);
     }
+  else
+    e = e ? e : hipMalloc(&nt->V, b64 * (prm->l_restart + 1));
   if (e != hipSuccess)
     {
       hps_newton_destroy(nt);
@@ -273,7 +407,8 @@ extern "C" int hps_newton_destroy(hps_newton nt)
     return HPS_OK;
   (void)hps_sync(nt->op);
   for (void *p : {(void *)nt->r, (void *)nt->inc, (void *)nt->tmp, (void *)nt->z64,
-                  (void *)nt->V, (void *)nt->x32, (void *)nt->y32, (void *)nt->z32})
+                  (void *)nt->V, (void *)nt->x32, (void *)nt->y32, (void *)nt->z32,
+                  (void *)nt->V32})
     if (p)
       (void)hipFree(p);
   delete nt;
@@ -335,7 +470,8 @@ extern "C" int hps_newton_solve_ref(hps_newton nt, double *x, const double *cons
           ++s.precond_updates;
         }
       int lin = 0;
-      int rc  = gmres(nt, nt->r, nt->inc, &lin);
+      int rc  = p.inner_fp32 ? gmres_mixed(nt, nt->r, nt->inc, &lin)
+                             : gmres(nt, nt->r, nt->inc, &lin);
       s.linear_iterations += lin;
       nt->history_linear_iterations_last = lin;
       if (rc != HPS_OK)

@@ -131,22 +131,42 @@ class MLP:
 
 
 class LearnedPredictor:
-    """Newton initial guess u_pred = 2 u_n - u_{n-1} + out_scale * MLP(features)."""
+    """Newton initial guess u_pred = 2 u_n - u_{n-1} + s_n * rel_scale * MLP(features).
 
-    def __init__(self, op, mlp: MLP = None, dt_scale=1.0, inc_scale=1.0, out_scale=None):
+    s_n is the RMS of the last increment u_n - u_{n-1} (a library norm); the
+    increment features are divided by s_n and the outputs multiplied by it,
+    so the network sees the same scales at every step of a trajectory."""
+
+    def __init__(self, op, mlp: MLP = None, dt_scale=1.0, rel_scale=None):
         self.op = op
         self.mlp = mlp or MLP(op)
         self.dt_scale = dt_scale
-        self.inc_scale = inc_scale
         dev = self.mlp.W[0].device
-        self.out_scale = (out_scale if out_scale is not None
-                          else torch.ones(N_OUT, dtype=torch.float32, device=dev))
+        self.rel_scale = (list(rel_scale) if rel_scale is not None else [1.0] * N_OUT)
+        self.out_scale = torch.ones(N_OUT, dtype=torch.float32, device=dev)
         n = op.n_nodes
         self.X = torch.empty(n, N_IN, dtype=torch.bfloat16, device=dev)
         self.top2 = torch.empty(n, 2, dtype=torch.int32, device=dev)
+        self.diff = torch.empty(op.n_dofs, dtype=torch.float64, device=dev)
         self.use_network = True
+        self.inc_scale = 1.0
+
+    def step_scale(self, u_prev, u_cur):
+        """RMS of u_cur - u_prev, computed by the library"""
+        h = self.op.handle
+        n = self.op.n_dofs
+        check(lib().hps_copy(h, n, _p(self.diff), _p(u_cur)))
+        check(lib().hps_axpy(h, n, -1.0, _p(u_prev), _p(self.diff)))
+        return max(self.op.norm(self.diff) / math.sqrt(n), 1e-300)
+
+    def _set_scales(self, u_prev, u_cur):
+        s = self.step_scale(u_prev, u_cur)
+        self.inc_scale = 1.0 / s
+        self.out_scale.copy_(torch.tensor([s * r for r in self.rel_scale], dtype=torch.float32))
+        return s
 
     def features(self, u_prev, u_cur, dt):
+        self._set_scales(u_prev, u_cur)
         check(lib().hps_mlp_features(self.op.handle, _p(u_prev), _p(u_cur), dt, self.dt_scale,
                                      self.inc_scale, _p(self.X), _p(self.top2)))
         return self.X, self.top2
@@ -161,6 +181,8 @@ class LearnedPredictor:
         return out
 
     def targets(self, u_prev, u_cur, u_next, T):
+        """(u_next - (2 u_cur - u_prev)) / out_scale on the 4 outputs; call
+        after features() of the same (u_prev, u_cur)"""
         check(lib().hps_mlp_targets(self.op.handle, _p(u_prev), _p(u_cur), _p(u_next),
                                     _p(self.top2), _p(self.out_scale), _p(T)))
         return T

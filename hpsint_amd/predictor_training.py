@@ -35,17 +35,20 @@ def _rms(x):
     return float(torch.sqrt(torch.mean(x.double() ** 2)))
 
 
-def scales(op, traj, dt):
-    """inc_scale, dt_scale, out_scale from a trajectory"""
+def relative_scales(op, traj):
+    """RMS over the trajectory of the second difference divided by the step
+    scale s_n = RMS(u_n - u_{n-1}), per output group (c, mu, eta, eta)"""
     nc = op.n_components()
-    inc = [traj[i] - traj[i - 1] for i in range(1, len(traj))]
-    inc_rms = math.sqrt(sum(_rms(d) ** 2 for d in inc) / len(inc))
-    sec = [(traj[i + 1] - 2 * traj[i] + traj[i - 1]).view(-1, nc) for i in range(1, len(traj) - 1)]
-    c = math.sqrt(sum(_rms(s[:, 0]) ** 2 for s in sec) / len(sec))
-    mu = math.sqrt(sum(_rms(s[:, 1]) ** 2 for s in sec) / len(sec))
-    eta = math.sqrt(sum(_rms(s[:, 2:]) ** 2 for s in sec) / len(sec))
-    out = [max(v, 1e-30) for v in (c, mu, eta, eta)]
-    return 1.0 / max(inc_rms, 1e-30), 1.0 / dt, out
+    acc = [0.0, 0.0, 0.0]
+    for i in range(1, len(traj) - 1):
+        s = max(_rms(traj[i] - traj[i - 1]), 1e-300)
+        sec = ((traj[i + 1] - 2 * traj[i] + traj[i - 1]) / s).view(-1, nc)
+        acc[0] += _rms(sec[:, 0]) ** 2
+        acc[1] += _rms(sec[:, 1]) ** 2
+        acc[2] += _rms(sec[:, 2:]) ** 2
+    m = max(len(traj) - 2, 1)
+    c, mu, eta = (max(math.sqrt(a / m), 1e-12) for a in acc)
+    return [c, mu, eta, eta]
 
 
 def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
@@ -69,12 +72,12 @@ def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
 
 
 def fit_predictor(op, traj, dt, iterations=1000, rows_per_step=32768, batch=16384,
-                  lr=1e-3, seed=0, log=None):
-    inc_scale, dt_scale, out_scale = scales(op, traj, dt)
+                  lr=1e-3, seed=0, log=None, skip=2):
+    """trains on the triples of traj[skip:] (the first steps of a run are the
+    relaxation of the initial condition and are left out)"""
+    traj = traj[skip:]
     mlp = MLP(op, seed=seed)
-    pred = LearnedPredictor(op, mlp, dt_scale=dt_scale, inc_scale=inc_scale,
-                            out_scale=torch.tensor(out_scale, dtype=torch.float32,
-                                                   device=mlp.W[0].device))
+    pred = LearnedPredictor(op, mlp, dt_scale=1.0 / dt, rel_scale=relative_scales(op, traj))
     X, T = build_dataset(pred, traj, dt, rows_per_step, seed)
     m = X.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed + 1)

@@ -160,6 +160,19 @@ int hps_convert_f64_to_f32(hps_operator op, int64_t n, float *dst,
 int hps_convert_f32_to_f64(hps_operator op, int64_t n, double *dst,
                            const float *src);
 
+/* FP32-storage vectors with FP64 accumulation (the Krylov basis of the
+ * mixed-precision GMRES): norms and Gram-Schmidt coefficients accumulate
+ * in FP64; updated entries are stored in FP32 */
+int hps_norm2_f32(hps_operator op, int64_t n, const float *x, double *result);
+int hps_multi_dot_f32(hps_operator op, int64_t n, int k, const float *V, const float *w,
+                      double *h);
+int hps_multi_axpy_norm_f32(hps_operator op, int64_t n, int k, const float *V,
+                            const double *h, float *w, double *norm);
+int hps_scale_f32(hps_operator op, int64_t n, double a, float *x);
+/* y (FP64) += sum_{j<k} a[j] X_j (FP32 vectors X + j*n), FP64 arithmetic */
+int hps_axpy_multi_f32(hps_operator op, int64_t n, int k, const double *a, const float *X,
+                       double *y);
+
 /* layout conversion: merge_components_fast / split_up_components_fast
  * (I/numerics/vector_tools.h:40-100); blocks is a host array of n_c device
  * pointers of n_nodes entries each */

@@ -164,10 +164,13 @@ def test_feature_kernel_matches_numpy():
     n = op.n_nodes
     up = rng.uniform(0, 1, n * (G + 2))
     uc = rng.uniform(0, 1, n * (G + 2))
-    pred = LearnedPredictor(op, inc_scale=37.0, dt_scale=500.0)
+    pred = LearnedPredictor(op, dt_scale=500.0)
     X, top2 = pred.features(torch.from_numpy(up).cuda(), torch.from_numpy(uc).cuda(), 1e-3)
     torch.cuda.synchronize()
-    Xr, topr = features_numpy(up, uc, (4, 5, 3), G + 2, 1e-3, 500.0, 37.0)
+    # the increment features are divided by the RMS of u_cur - u_prev
+    s = np.sqrt(np.mean((uc - up) ** 2))
+    assert abs(pred.inc_scale * s - 1.0) < 1e-13
+    Xr, topr = features_numpy(up, uc, (4, 5, 3), G + 2, 1e-3, 500.0, pred.inc_scale)
     np.testing.assert_array_equal(top2.cpu().numpy(), topr)
     # the kernel rounds float64 -> float32 -> bf16; reproduce that exactly
     want = mo.bf16(Xr.astype(np.float32))

@@ -253,3 +253,32 @@ def test_vmult_and_residual_tile_geometries(monkeypatch, ty, tz):
     ref = po.residual(p, 2, u, [old])
     op.evaluate_nonlinear_residual(dst, dev(u), 2, old=[dev(old)])
     assert rel_l2(host(dst), ref) <= TOL64
+
+
+def test_fp32_storage_blas_accumulates_in_fp64():
+    mesh = CartesianMesh((7, 5, 3), (0, 0, 0), (7.0, 5.0, 3.0))
+    op = SinteringOperatorGeneric(mesh, 3)
+    h, L, n = op.handle, _ffi.lib(), op.n_dofs
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    rng = np.random.default_rng(1)
+    k = 5
+    V = rng.standard_normal((k, n)).astype(np.float32)
+    w = rng.standard_normal(n).astype(np.float32)
+    V64, w64 = V.astype(np.float64), w.astype(np.float64)
+    dV, dw = dev(V.ravel(), torch.float32), dev(w, torch.float32)
+    r = ctypes.c_double()
+    _ffi.check(L.hps_norm2_f32(h, n, P(dw), ctypes.byref(r)))
+    assert abs(r.value - np.linalg.norm(w64)) <= 1e-13 * np.linalg.norm(w64)
+    hh = (ctypes.c_double * k)()
+    _ffi.check(L.hps_multi_dot_f32(h, n, k, P(dV), P(dw), hh))
+    np.testing.assert_allclose(list(hh), V64 @ w64, rtol=1e-12, atol=1e-12)
+    _ffi.check(L.hps_multi_axpy_norm_f32(h, n, k, P(dV), hh, P(dw), ctypes.byref(r)))
+    wr = (w64 - V64.T @ (V64 @ w64)).astype(np.float32)
+    np.testing.assert_allclose(host(dw), wr, rtol=1e-6, atol=1e-6)
+    assert abs(r.value - np.linalg.norm(wr.astype(np.float64))) <= 1e-12 * np.linalg.norm(wr)
+    y = dev(np.zeros(n))
+    a = (ctypes.c_double * k)(*[0.5, -1.0, 2.0, 0.25, 3.0])
+    _ffi.check(L.hps_axpy_multi_f32(h, n, k, a, P(dV), P(y)))
+    np.testing.assert_allclose(host(y), np.array(list(a)) @ V64, rtol=1e-14, atol=1e-14)
+    _ffi.check(L.hps_scale_f32(h, n, 0.5, P(dw)))
+    np.testing.assert_allclose(host(dw), wr * np.float32(0.5), rtol=0, atol=0)
