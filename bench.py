@@ -37,7 +37,78 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--timesteps", type=int, default=5,
+                    help="held-out timesteps timed per method (0 = skip)")
+    ap.add_argument("--train-steps", type=int, default=15,
+                    help="plain-solver steps that train the predictor")
+    ap.add_argument("--train-iterations", type=int, default=1500)
     return ap.parse_args()
+
+
+def timestep_comparison(op, u0, dt, args):
+    """s/timestep of plain FP64 Newton-GMRES vs FP32 inner Krylov + learned
+    predictor, Newton rtol 1e-8 (relative to |R(u_n)|, the plain method's
+    initial residual), GMRES rtol 1e-2, node-block Jacobi"""
+    import time
+    import torch
+    from hpsint_amd.predictor_training import fit_predictor
+    from hpsint_amd.solvers import NewtonSolver, TimeStepper
+
+    def stepper(fp32, predictor=None):
+        solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
+                              preconditioner="InverseComponentBlockDiagonalMatrix",
+                              inner_fp32=fp32)
+        return TimeStepper(op, solver, order=1, dt=dt, predictor=predictor)
+
+    # plain FP64 trajectory: the training data, and the plain timings of the
+    # steps that follow it
+    ts = stepper(False)
+    ts.initialize(u0)
+    traj = [u0.clone()]
+    recs = []
+    for _ in range(args.train_steps + args.timesteps):
+        recs.append(ts.make_step())
+        traj.append(ts.u.clone())
+    plain = recs[args.train_steps:]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pred = fit_predictor(op, traj[:args.train_steps + 1], dt,
+                         iterations=args.train_iterations,
+                         rows_per_step=min(32768, op.n_nodes), seed=0)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+
+    def run(fp32, predictor):
+        s = stepper(fp32, predictor)
+        k = args.train_steps
+        s.initialize(traj[k])
+        s.history = [traj[k].clone(), traj[k - 1].clone()]
+        s.t = k * dt
+        out = [s.make_step() for _ in range(args.timesteps)]
+        err = float(torch.linalg.vector_norm(s.u - traj[-1]) / torch.linalg.vector_norm(traj[-1]))
+        return out, err
+
+    mp, err_mp = run(True, pred)
+    mp_only, _ = run(True, None)
+    learned_only, _ = run(False, pred)
+
+    def summary(rs):
+        return {"s_per_step": sum(r.seconds for r in rs) / len(rs),
+                "newton": [r.newton_iterations for r in rs],
+                "linear": [r.linear_iterations for r in rs]}
+    sp, sm = summary(plain), summary(mp)
+    return {
+        "workload": "%d plain FP64 steps (BDF1, dt=%g) train the predictor; the next %d "
+                    "steps are timed per method" % (args.train_steps, dt, args.timesteps),
+        "fp64_plain": sp,
+        "mp_learned": sm,
+        "mp_only": summary(mp_only),
+        "fp64_learned": summary(learned_only),
+        "speedup_mp_learned_vs_fp64": sp["s_per_step"] / sm["s_per_step"],
+        "final_state_rel_diff_mp_learned_vs_fp64": err_mp,
+        "predictor_training_s": t_train,
+        "predictor_final_loss": pred.losses[-1],
+    }
 
 
 def main():
@@ -152,6 +223,10 @@ def main():
                          "oracle (direct per-cell quadrature, OpenMP over colour classes; "
                          "CPU restatement, not deal.II) on %s" % (reps, _cpu_model())}
 
+    steps = None
+    if args.timesteps > 0:
+        steps = timestep_comparison(op, u, dt, args)
+
     out = {
         "metric": "tangent vmult GDoF/s (% HBM roofline); s/timestep FP64 vs MP+learned",
         "value": n_dofs / t_vmult / 1e9,
@@ -184,6 +259,7 @@ def main():
         "residual_gdofs": n_dofs / t_res / 1e9,
         "residual_hbm_frac": res_achieved / HBM_PEAK_GBS,
         "vmult_f32_gdofs": n_dofs / t_vm32 / 1e9,
+        "timesteps": steps,
     }
     print(json.dumps(out))
 

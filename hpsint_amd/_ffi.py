@@ -152,9 +152,9 @@ _SIGNATURES = {
     "hps_mlp_loss": [vp, i64, vp, vp, vp, vp, cint, ctypes.POINTER(ctypes.c_float)],
     "hps_mlp_adam": [vp, cint, cint, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
                      ctypes.c_float, ctypes.c_float, cint, vp],
-    "hps_mlp_features": [vp, vp, vp, dbl, dbl, dbl, vp, vp],
-    "hps_mlp_predict": [vp, vp, vp, vp, vp, vp, vp],
-    "hps_mlp_targets": [vp, vp, vp, vp, vp, vp, vp],
+    "hps_mlp_features": [vp, vp, vp, dbl, dbl, dbl, vp, vp, vp],
+    "hps_mlp_predict": [vp, vp, vp, vp, vp, vp, vp, vp],
+    "hps_mlp_targets": [vp, vp, vp, vp, vp, vp, vp, vp],
 }
 
 

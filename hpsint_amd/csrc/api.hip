@@ -60,7 +60,10 @@ struct hps_operator_s
   hps_params           p;
   int                  dim, n_grains, nc;
   bool                 tens;
-  const hps::OpTable  *ops;
+  int                  degree;
+  const hps::OpTable  *ops; // Q1 kernels
+  const hps::Q2Table  *q2;  // Q2 kernels
+  hps::Q2Params        qp;
   hps::KParams         kp;
   hps::FixParams       fp;
   int                  n_tiles, tz;
@@ -89,6 +92,17 @@ static int check_op(hps_operator op)
   return HPS_OK;
 }
 
+// Q2 launch parameters with the current time weights
+static hps::Q2Params q2_params(hps_operator op)
+{
+  hps::Q2Params q = op->qp;
+  q.pp            = op->kp.pp;
+  q.order         = op->kp.order;
+  for (int i = 0; i < 3; ++i)
+    q.w_hist[i] = op->kp.w_hist[i];
+  return q;
+}
+
 int hps_internal_fail(int code, const char *msg)
 {
   return fail(code, msg);
@@ -106,8 +120,9 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   *out = nullptr;
   if (prm->dim != 2 && prm->dim != 3)
     return fail(HPS_ERR_UNSUPPORTED, "dim must be 2 or 3");
-  if (prm->fe_degree != 1)
-    return fail(HPS_ERR_UNSUPPORTED, "only FE_Q(1) with QGauss(2) is built");
+  if (prm->fe_degree != 1 && prm->fe_degree != 2)
+    return fail(HPS_ERR_UNSUPPORTED,
+                "only FE_Q(1) with QGauss(2) and FE_Q(2) with QGauss(3) are built");
   if (prm->mobility != HPS_MOBILITY_SCALAR && prm->mobility != HPS_MOBILITY_TENSORIAL)
     return fail(HPS_ERR_INVALID_ARGUMENT, "unknown mobility");
   for (int d = 0; d < prm->dim; ++d)
@@ -121,12 +136,17 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   const int TZ     = env_int("HPSINT_AMD_TILE_M", 16);
   if (TZ < 1 || TZ > 4096)
     return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_TILE_M out of range");
-  const hps::OpTable *ops =
-    prm->dim == 3 ? hps::q1_table_d3(G, tens, tile_y) : hps::q1_table_d2(G, tens);
-  if (!ops)
+  const bool          q2  = prm->fe_degree == 2;
+  const hps::OpTable *ops = nullptr;
+  const hps::Q2Table *q2t = nullptr;
+  if (q2)
+    q2t = hps::q2_table(prm->dim, G, tens);
+  else
+    ops = prm->dim == 3 ? hps::q1_table_d3(G, tens, tile_y) : hps::q1_table_d2(G, tens);
+  if (!ops && !q2t)
     return fail(HPS_ERR_INVALID_N_COMP,
                 "n_components = " + std::to_string(prm->n_components) +
-                  " is not instantiated for this dim/mobility");
+                  " is not instantiated for this dim/degree/mobility");
 
   hps_operator op = new hps_operator_s();
   std::memset((void *)op, 0, sizeof(*op));
@@ -135,23 +155,29 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   op->n_grains     = G;
   op->nc           = G + 2;
   op->tens         = tens;
+  op->degree       = prm->fe_degree;
   op->ops          = ops;
+  op->q2           = q2t;
   op->precond_kind = -1;
 
   hps::KParams &kp = op->kp;
   const int     ncx = prm->n_cells[0];
   const int     ncy = prm->dim == 3 ? prm->n_cells[1] : 1;
   const int     ncm = prm->dim == 3 ? prm->n_cells[2] : prm->n_cells[1];
+  const int     p   = prm->fe_degree;
   kp.nc[0]         = ncx;
   kp.nc[1]         = ncy;
   kp.nc[2]         = ncm;
-  kp.nn[0]         = ncx + 1;
-  kp.nn[1]         = prm->dim == 3 ? ncy + 1 : 1;
-  kp.nn[2]         = ncm + 1;
-  kp.nt[0]         = (ncx + ops->tx - 1) / ops->tx;
-  kp.nt[1]         = (ncy + ops->ty - 1) / ops->ty;
-  kp.nt[2]         = (ncm + TZ - 1) / TZ;
+  kp.nn[0]         = p * ncx + 1;
+  kp.nn[1]         = prm->dim == 3 ? p * ncy + 1 : 1;
+  kp.nn[2]         = p * ncm + 1;
   kp.tz            = TZ;
+  if (ops)
+    {
+      kp.nt[0] = (ncx + ops->tx - 1) / ops->tx;
+      kp.nt[1] = (ncy + ops->ty - 1) / ops->ty;
+      kp.nt[2] = (ncm + TZ - 1) / TZ;
+    }
   double vol       = 1.0;
   for (int d = 0; d < 3; ++d)
     kp.inv_h[d] = d < prm->dim ? 1.0 / prm->cell_size[d] : 1.0;
@@ -171,33 +197,8 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   kp.pp.w_alg   = 1.0;
   kp.order      = 0;
 
-  op->n_tiles = kp.nt[0] * kp.nt[1] * kp.nt[2];
   op->n_nodes = (int64_t)kp.nn[0] * kp.nn[1] * kp.nn[2];
   op->n_dofs  = op->n_nodes * op->nc;
-
-  const int px = ops->tx + 1;
-  const int pn = px * (prm->dim == 3 ? ops->ty + 1 : 1);
-  op->cache_elems = (int64_t)op->n_tiles * TZ * ops->n_fields * ops->nt;
-  op->tz          = TZ;
-  op->part_elems  = (int64_t)op->n_tiles * (TZ + 1) * pn * op->nc;
-
-  hps::FixParams &fp = op->fp;
-  for (int d = 0; d < 3; ++d)
-    {
-      fp.nc[d] = kp.nc[d];
-      fp.nn[d] = kp.nn[d];
-      fp.nt[d] = kp.nt[d];
-      fp.nb[d] = kp.nt[d] - 1;
-    }
-  fp.tsz[0] = ops->tx;
-  fp.tsz[1] = prm->dim == 3 ? ops->ty : 1;
-  fp.tsz[2] = TZ;
-  fp.n_comp = op->nc;
-  fp.px     = px;
-  fp.pn     = pn;
-  fp.n_m    = (int64_t)fp.nb[2] * fp.nn[0] * fp.nn[1];
-  fp.n_y    = (int64_t)fp.nb[1] * fp.nn[0] * fp.nn[2];
-  fp.n_x    = (int64_t)fp.nb[0] * fp.nn[1] * fp.nn[2];
 
   auto cleanup_fail = [&](hipError_t e) {
     hps_destroy(op);
@@ -205,8 +206,54 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
                 std::string("allocation: ") + hipGetErrorString(e));
   };
   hipError_t e;
-  if ((e = hipMalloc(&op->part, op->part_elems * sizeof(double))) != hipSuccess)
-    return cleanup_fail(e);
+  if (q2)
+    {
+      // Q2 geometry: physical x, y, z (z = 1 cell in 2D)
+      hps::Q2Params &qp = op->qp;
+      qp.nc[0]          = prm->n_cells[0];
+      qp.nc[1]          = prm->n_cells[1];
+      qp.nc[2]          = prm->dim == 3 ? prm->n_cells[2] : 1;
+      qp.h_prod         = 1.0;
+      for (int d = 0; d < 3; ++d)
+        {
+          qp.nn[d]    = d < prm->dim ? 2 * qp.nc[d] + 1 : 1;
+          qp.inv_h[d] = d < prm->dim ? 1.0 / prm->cell_size[d] : 1.0;
+          if (d < prm->dim)
+            qp.h_prod *= prm->cell_size[d];
+        }
+      op->cache_elems =
+        hps::q2_workgroups(qp, prm->dim, q2t->cpw) * (int64_t)q2t->n_fields * 64;
+      op->part_elems = 0;
+    }
+  else
+    {
+      op->n_tiles = kp.nt[0] * kp.nt[1] * kp.nt[2];
+      const int px = ops->tx + 1;
+      const int pn = px * (prm->dim == 3 ? ops->ty + 1 : 1);
+      op->cache_elems = (int64_t)op->n_tiles * TZ * ops->n_fields * ops->nt;
+      op->tz          = TZ;
+      op->part_elems  = (int64_t)op->n_tiles * (TZ + 1) * pn * op->nc;
+
+      hps::FixParams &fp = op->fp;
+      for (int d = 0; d < 3; ++d)
+        {
+          fp.nc[d] = kp.nc[d];
+          fp.nn[d] = kp.nn[d];
+          fp.nt[d] = kp.nt[d];
+          fp.nb[d] = kp.nt[d] - 1;
+        }
+      fp.tsz[0] = ops->tx;
+      fp.tsz[1] = prm->dim == 3 ? ops->ty : 1;
+      fp.tsz[2] = TZ;
+      fp.n_comp = op->nc;
+      fp.px     = px;
+      fp.pn     = pn;
+      fp.n_m    = (int64_t)fp.nb[2] * fp.nn[0] * fp.nn[1];
+      fp.n_y    = (int64_t)fp.nb[1] * fp.nn[0] * fp.nn[2];
+      fp.n_x    = (int64_t)fp.nb[0] * fp.nn[1] * fp.nn[2];
+      if ((e = hipMalloc(&op->part, op->part_elems * sizeof(double))) != hipSuccess)
+        return cleanup_fail(e);
+    }
   const int64_t red = (int64_t)hps::RED_BLOCKS * (hps::MAXK > op->nc ? hps::MAXK : op->nc);
   if ((e = hipMalloc(&op->red_partial, red * sizeof(double))) != hipSuccess)
     return cleanup_fail(e);
@@ -309,15 +356,22 @@ extern "C" int hps_set_linearization(hps_operator op, const double *u)
       HPS_HIP(hipMalloc(&op->cache64, op->cache_elems * sizeof(double)));
       HPS_HIP(hipMalloc(&op->cache32, op->cache_elems * sizeof(float)));
     }
-  op->ops->fill(op->kp, op->n_tiles, u, op->cache64, op->cache32, op->stream);
+  if (op->q2)
+    op->q2->fill(q2_params(op), u, op->cache64, op->cache32, op->stream);
+  else
+    op->ops->fill(op->kp, op->n_tiles, u, op->cache64, op->cache32, op->stream);
   HPS_LAUNCHED();
   op->lin_set = true;
   return HPS_OK;
 }
 
+// sums the tile-face partials of the Q1 kernels (the Q2 colour passes
+// complete dst themselves)
 template <typename T>
 static int finish_fixup(hps_operator op, T *dst, int ostride, int ocstride, int ooffset)
 {
+  if (op->q2)
+    return HPS_OK;
   hps::FixParams fp = op->fp;
   fp.ostride        = ostride;
   fp.ocstride       = ocstride;
@@ -350,7 +404,11 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
   // with order 0 there is no history: evaluate without the history term
   const int w = (wtd == 2 && op->kp.order == 0) ? 1 : wtd;
-  op->ops->residual(op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part, op->stream);
+  if (op->q2)
+    op->q2->residual(q2_params(op), w, src, o[0], o[1], o[2], dst, op->stream);
+  else
+    op->ops->residual(op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part,
+                      op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[1], op->stream));
@@ -371,7 +429,10 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
     return fail(HPS_ERR_NOT_INITIALIZED, "vmult before hps_set_linearization");
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
-  op->ops->vmult64(op->kp, op->n_tiles, src, dst, op->part, op->cache64, op->stream);
+  if (op->q2)
+    op->q2->vmult64(q2_params(op), src, dst, op->cache64, op->stream);
+  else
+    op->ops->vmult64(op->kp, op->n_tiles, src, dst, op->part, op->cache64, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[1], op->stream));
@@ -392,7 +453,11 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
     return fail(HPS_ERR_NOT_INITIALIZED, "vmult before hps_set_linearization");
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
-  op->ops->vmult32(op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32, op->stream);
+  if (op->q2)
+    op->q2->vmult32(q2_params(op), src, dst, op->cache32, op->stream);
+  else
+    op->ops->vmult32(op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32,
+                     op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[1], op->stream));
@@ -419,8 +484,11 @@ extern "C" int hps_precond_update(hps_operator op, int kind)
           HPS_HIP(hipMalloc(&op->inv_diag64, n * sizeof(double)));
           HPS_HIP(hipMalloc(&op->inv_diag32, n * sizeof(float)));
         }
-      op->ops->probe(op->kp, op->n_tiles, op->cache64, op->diag, op->part, -1, nc, 1, 0,
-                     op->stream);
+      if (op->q2)
+        op->q2->probe(q2_params(op), op->cache64, op->diag, -1, nc, 1, 0, op->stream);
+      else
+        op->ops->probe(op->kp, op->n_tiles, op->cache64, op->diag, op->part, -1, nc, 1, 0,
+                       op->stream);
       HPS_LAUNCHED();
       if (int r = finish_fixup<double>(op, op->diag, nc, 1, 0))
         return r;
@@ -437,8 +505,12 @@ extern "C" int hps_precond_update(hps_operator op, int kind)
         }
       for (int j = 0; j < nc; ++j)
         {
-          op->ops->probe(op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc, nc,
-                         j, op->stream);
+          if (op->q2)
+            op->q2->probe(q2_params(op), op->cache64, op->blocks, j, nc * nc, nc, j,
+                          op->stream);
+          else
+            op->ops->probe(op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc,
+                           nc, j, op->stream);
           HPS_LAUNCHED();
           if (int r = finish_fixup<double>(op, op->blocks, nc * nc, nc, j))
             return r;
@@ -795,7 +867,8 @@ extern "C" int hps_mlp_adam(hps_operator op, int n_out, int n_in, float *W, floa
 }
 
 extern "C" int hps_mlp_features(hps_operator op, const double *u_prev, const double *u_cur,
-                                double dt, double dt_scale, double inc_scale, void *X, int *top2)
+                                double dt, double dt_scale, double inc_scale, void *X, int *top2,
+                                float *lscale)
 {
   if (int r = check_op(op))
     return r;
@@ -815,29 +888,32 @@ extern "C" int hps_mlp_features(hps_operator op, const double *u_prev, const dou
   fp.dt        = dt;
   fp.dt_scale  = dt_scale;
   fp.inc_scale = inc_scale;
-  hps::mlp_features(op->stream, fp, u_prev, u_cur, (uint16_t *)X, top2);
+  fp.inc_floor = inc_scale > 0.0 ? 1e-3 / inc_scale : 0.0;
+  hps::mlp_features(op->stream, fp, u_prev, u_cur, (uint16_t *)X, top2, lscale);
   HPS_LAUNCHED();
   return HPS_OK;
 }
 
 extern "C" int hps_mlp_predict(hps_operator op, const double *u_prev, const double *u_cur,
                                const float *Y, const int *top2, const float *out_scale,
-                               double *u_pred)
+                               const float *lscale, double *u_pred)
 {
   if (int r = check_op(op))
     return r;
-  hps::mlp_predict(op->stream, op->n_nodes, op->nc, u_prev, u_cur, Y, top2, out_scale, u_pred);
+  hps::mlp_predict(op->stream, op->n_nodes, op->nc, u_prev, u_cur, Y, top2, out_scale, lscale,
+                   u_pred);
   HPS_LAUNCHED();
   return HPS_OK;
 }
 
 extern "C" int hps_mlp_targets(hps_operator op, const double *u_prev, const double *u_cur,
                                const double *u_next, const int *top2, const float *out_scale,
-                               float *T)
+                               const float *lscale, float *T)
 {
   if (int r = check_op(op))
     return r;
-  hps::mlp_targets(op->stream, op->n_nodes, op->nc, u_prev, u_cur, u_next, top2, out_scale, T);
+  hps::mlp_targets(op->stream, op->n_nodes, op->nc, u_prev, u_cur, u_next, top2, out_scale,
+                   lscale, T);
   HPS_LAUNCHED();
   return HPS_OK;
 }

@@ -1,0 +1,664 @@
+// fe_q2.hpp -- Q2 cell kernels (3-point Gauss rule per direction) of the
+// sintering operator for gfx950.
+//
+// Same operations as the Q1 kernels (tangent vmult, residual, linearisation
+// cache, diagonal / node-block probing; S/operator_sintering_generic.h:
+// 539-577, 877-1006, S/operator_sintering_data.h:301-409), for FE_Q(2) with
+// QGauss(3) on a uniform Cartesian mesh.
+//
+// Lane mapping.  A workgroup is one wave; its lanes are split into cells of
+// NP = 3^DIM lanes (2 cells per wave in 3D, 7 in 2D).  Lane p of a cell,
+// p = t0 + 3 t1 + 9 t2, stands for node (t0, t1, t2) of the cell on gather
+// and scatter and for Gauss point (t0, t1, t2) in between.  Sum
+// factorisation runs through LDS, one pass per direction: a lane reads the
+// three entries of its line along d and combines them with one row (or, to
+// integrate, one column) of a 3x3 matrix:
+//   S[q][n]  = L_n(x_q), Q2 Lagrange basis (nodes 0, 1/2, 1) at the Gauss
+//              points x_q = 1/2 - sqrt(15)/10, 1/2, 1/2 + sqrt(15)/10;
+//   Dq[q][r] = l_r'(x_q), derivative of the Lagrange basis through the
+//              Gauss points (collocation derivative: gradients at the
+//              Gauss points from the values at the Gauss points, exact for
+//              the Q2 field because Dq S = L').
+// Interpolation: values = S_x S_y S_z u, gradient_d = Dq_d (values) / h_d.
+// Integration:   y = JxW v + sum_d Dq_d^T (JxW g_d / h_d),  r = S^T y.
+//
+// Assembly.  Cells are processed in 2^DIM colour passes (parity of the cell
+// index along every direction); cells of one colour share no node, so a
+// pass updates dst without atomics.  The first colour that reaches a node
+// stores, later colours add, always in colour order: every entry is summed
+// in a fixed order (deterministic).  A cell is the first to reach its local
+// node (t0, t1, t2) iff along every direction its index is even, the node is
+// the cell's middle node, or the neighbour cell sharing the node does not
+// exist.
+//
+// The linearisation cache is stored per workgroup and field in lane order,
+// cache[(colour base + workgroup) * NF + field][64], one coalesced 512-byte
+// read per field.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "physics.hpp"
+
+namespace hps
+{
+  struct Q2Params
+  {
+    int        nc[3];   // cells along x, y, z (1 along z in 2D)
+    int        nn[3];   // nodes along x, y, z: 2 nc + 1 (1 along z in 2D)
+    int        par[3];  // colour of this launch: parity of the cell index
+    int        ncol[3]; // cells of this colour along x, y, z
+    int64_t    n_col;   // cells of this colour
+    int64_t    wg_base; // first workgroup of this colour in the cache
+    int        order;   // number of history vectors (BDF order)
+    double     inv_h[3];
+    double     h_prod;    // cell volume
+    double     w_hist[3]; // BDF weights[1..order]
+    PhysParams pp;
+  };
+
+  template <typename T>
+  struct Q2Coef
+  {
+    // S rows: (s_a, s_b, s_c), (0, 1, 0), (s_c, s_b, s_a)
+    static constexpr T s_a = T(0.68729833462074168851792653997823996108);
+    static constexpr T s_b = T(0.4);
+    static constexpr T s_c = T(-0.087298334620741688517926539978239961);
+    // Dq rows: (d_a, d_b, d_c), (d_c, 0, -d_c), (-d_c, -d_b, -d_a)
+    static constexpr T d_a = T(-3.8729833462074168851792653997823996108);
+    static constexpr T d_b = T(5.1639777949432225135723538663765328144);
+    static constexpr T d_c = T(-1.2909944487358056283930884665941332036);
+    // L' rows (derivative of the Q2 basis at the Gauss points):
+    // (e_a, e_b, e_c), (-1, 0, 1), (-e_c, -e_b, -e_a)
+    static constexpr T e_a = T(-2.5491933384829667540717061599129598443);
+    static constexpr T e_b = T(3.0983866769659335081434123198259196887);
+    static constexpr T e_c = T(-0.5491933384829667540717061599129598443);
+    // Gauss weights on [0, 1]
+    static constexpr T w_a = T(0.27777777777777777777777777777777777778);
+    static constexpr T w_b = T(0.44444444444444444444444444444444444444);
+  };
+
+  template <typename T>
+  __device__ __forceinline__ T sel3(int t, T a, T b, T c)
+  {
+    return t == 0 ? a : (t == 1 ? b : c);
+  }
+
+  // row t of S: out[n] = S[t][n]
+  template <typename T>
+  __device__ __forceinline__ void q2_s_row(int t, T *r)
+  {
+    using C = Q2Coef<T>;
+    r[0]    = sel3(t, C::s_a, T(0), C::s_c);
+    r[1]    = t == 1 ? T(1) : C::s_b;
+    r[2]    = sel3(t, C::s_c, T(0), C::s_a);
+  }
+  // column t of S: out[q] = S[q][t]
+  template <typename T>
+  __device__ __forceinline__ void q2_s_col(int t, T *r)
+  {
+    using C = Q2Coef<T>;
+    r[0]    = sel3(t, C::s_a, C::s_b, C::s_c);
+    r[1]    = t == 1 ? T(1) : T(0);
+    r[2]    = sel3(t, C::s_c, C::s_b, C::s_a);
+  }
+  // row t of Dq
+  template <typename T>
+  __device__ __forceinline__ void q2_dq_row(int t, T *r)
+  {
+    using C = Q2Coef<T>;
+    r[0]    = sel3(t, C::d_a, C::d_c, -C::d_c);
+    r[1]    = sel3(t, C::d_b, T(0), -C::d_b);
+    r[2]    = sel3(t, C::d_c, -C::d_c, -C::d_a);
+  }
+  // column t of Dq
+  template <typename T>
+  __device__ __forceinline__ void q2_dq_col(int t, T *r)
+  {
+    using C = Q2Coef<T>;
+    r[0]    = sel3(t, C::d_a, C::d_b, C::d_c);
+    r[1]    = sel3(t, C::d_c, T(0), -C::d_c);
+    r[2]    = sel3(t, -C::d_c, -C::d_b, -C::d_a);
+  }
+  // row t of L'
+  template <typename T>
+  __device__ __forceinline__ void q2_e_row(int t, T *r)
+  {
+    using C = Q2Coef<T>;
+    r[0]    = sel3(t, C::e_a, T(-1), -C::e_c);
+    r[1]    = sel3(t, C::e_b, T(0), -C::e_b);
+    r[2]    = sel3(t, C::e_c, T(1), -C::e_a);
+  }
+  template <typename T>
+  __device__ __forceinline__ T q2_w(int t)
+  {
+    return t == 1 ? Q2Coef<T>::w_b : Q2Coef<T>::w_a;
+  }
+
+  // out = sum_a coef[a] * buf[base + a * str]
+  template <typename T>
+  __device__ __forceinline__ T q2_pass(const T *buf, int base, int str, const T *coef)
+  {
+    return coef[0] * buf[base] + coef[1] * buf[base + str] + coef[2] * buf[base + 2 * str];
+  }
+
+  template <int DIM>
+  struct Q2Lane
+  {
+    static constexpr int NP  = DIM == 3 ? 27 : 9;
+    static constexpr int CPW = 64 / NP; // cells per wave
+    static constexpr int str(int d)
+    {
+      return d == 0 ? 1 : (d == 1 ? 3 : 9);
+    }
+
+    int  l, s, p, t[3];
+    bool valid; // lane belongs to an existing cell of this colour
+    int  cc[3]; // its cell
+    int  b0[3]; // first lane of this lane's line along x, y, z
+
+    __device__ Q2Lane(const Q2Params &kp)
+    {
+      l     = threadIdx.x;
+      s     = l / NP;
+      p     = l - s * NP;
+      t[0]  = p % 3;
+      t[1]  = (p / 3) % 3;
+      t[2]  = DIM == 3 ? p / 9 : 0;
+      valid = s < CPW && cell(kp, (int64_t)blockIdx.x * CPW + s, cc);
+      // lanes past the last cell slot read the first cell's lines, so every
+      // LDS read stays inside the 64 entries of a row
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        b0[d] = s < CPW ? l - t[d] * str(d) : 0;
+    }
+    __device__ int base(int d) const
+    {
+      return b0[d];
+    }
+
+    // cell coordinates of the ci-th cell of the launch's colour
+    __device__ static bool cell(const Q2Params &kp, int64_t ci, int *c)
+    {
+      if (ci >= kp.n_col)
+        return false;
+      const int64_t q  = ci / kp.ncol[0];
+      const int     ix = (int)(ci - q * kp.ncol[0]);
+      const int     iy = (int)(q % kp.ncol[1]);
+      const int     iz = (int)(q / kp.ncol[1]);
+      c[0]             = 2 * ix + kp.par[0];
+      c[1]             = 2 * iy + kp.par[1];
+      c[2]             = 2 * iz + kp.par[2];
+      return true;
+    }
+    __device__ static int64_t node(const Q2Params &kp, const int *c, int p)
+    {
+      const int gx = 2 * c[0] + p % 3;
+      const int gy = 2 * c[1] + (p / 3) % 3;
+      const int gz = DIM == 3 ? 2 * c[2] + p / 9 : 0;
+      return gx + (int64_t)kp.nn[0] * (gy + (int64_t)kp.nn[1] * gz);
+    }
+    // this cell is the first of the colour sequence to reach local node p
+    __device__ static bool first(const Q2Params &kp, const int *c, int p)
+    {
+      const int tt[3] = {p % 3, (p / 3) % 3, DIM == 3 ? p / 9 : 1};
+      bool      f     = true;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        f = f && ((c[d] & 1) == 0 || tt[d] == 1 || (tt[d] == 0 && c[d] == 0) ||
+                  (tt[d] == 2 && c[d] == kp.nc[d] - 1));
+      return f;
+    }
+  };
+
+  // values and gradients at this lane's Gauss point of the field whose
+  // nodal values (lane order) are in u; w0..w2: scratch rows of 64
+  template <int DIM, typename T>
+  __device__ __forceinline__ void q2_interp_vg(const Q2Lane<DIM> &L, const T *u, T *w0, T *w1,
+                                               T *w2, const T (*srow)[3], const T (*drow)[3],
+                                               const T *inv_h, T &val, T *grad)
+  {
+    const int l = L.l;
+    if constexpr (DIM == 3)
+      {
+        w0[l] = q2_pass(u, L.base(0), 1, srow[0]);
+        __syncthreads();
+        w1[l] = q2_pass(w0, L.base(1), 3, srow[1]);
+        __syncthreads();
+        val   = q2_pass(w1, L.base(2), 9, srow[2]);
+        w2[l] = val;
+        __syncthreads();
+      }
+    else
+      {
+        w0[l] = q2_pass(u, L.base(0), 1, srow[0]);
+        __syncthreads();
+        val   = q2_pass(w0, L.base(1), 3, srow[1]);
+        w2[l] = val;
+        __syncthreads();
+      }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      grad[d] = q2_pass(w2, L.base(d), Q2Lane<DIM>::str(d), drow[d]) * inv_h[d];
+  }
+
+  // values only
+  template <int DIM, typename T>
+  __device__ __forceinline__ T q2_interp_v(const Q2Lane<DIM> &L, const T *u, T *w0, T *w1,
+                                           const T (*srow)[3])
+  {
+    const int l = L.l;
+    w0[l]       = q2_pass(u, L.base(0), 1, srow[0]);
+    __syncthreads();
+    if constexpr (DIM == 3)
+      {
+        w1[l] = q2_pass(w0, L.base(1), 3, srow[1]);
+        __syncthreads();
+        const T v = q2_pass(w1, L.base(2), 9, srow[2]);
+        __syncthreads();
+        return v;
+      }
+    else
+      {
+        const T v = q2_pass(w0, L.base(1), 3, srow[1]);
+        __syncthreads();
+        return v;
+      }
+  }
+
+  // nodal contribution of this lane's node from the Gauss-point value
+  // coefficient vq and gradient coefficients gq[d] (JxW and 1/h applied)
+  template <int DIM, typename T>
+  __device__ __forceinline__ T q2_integrate(const Q2Lane<DIM> &L, T vq, const T *gq, T *w0,
+                                            T *w1, T *w2, T *w3, const T (*scol)[3],
+                                            const T (*dcol)[3])
+  {
+    const int l = L.l;
+    w0[l]       = gq[0];
+    w1[l]       = gq[1];
+    if constexpr (DIM == 3)
+      w2[l] = gq[2];
+    __syncthreads();
+    T y = vq;
+    y += q2_pass(w0, L.base(0), 1, dcol[0]);
+    y += q2_pass(w1, L.base(1), 3, dcol[1]);
+    if constexpr (DIM == 3)
+      y += q2_pass(w2, L.base(2), 9, dcol[2]);
+    w3[l] = y;
+    __syncthreads();
+    T r;
+    if constexpr (DIM == 3)
+      {
+        w0[l] = q2_pass(w3, L.base(2), 9, scol[2]);
+        __syncthreads();
+        w1[l] = q2_pass(w0, L.base(1), 3, scol[1]);
+        __syncthreads();
+        r = q2_pass(w1, L.base(0), 1, scol[0]);
+      }
+    else
+      {
+        w0[l] = q2_pass(w3, L.base(1), 3, scol[1]);
+        __syncthreads();
+        r = q2_pass(w0, L.base(0), 1, scol[0]);
+      }
+    __syncthreads();
+    return r;
+  }
+
+  // gather n_comp components of the cells of this workgroup into
+  // buf[c * 64 + lane] (lane order); missing cells read as zero
+  template <int DIM, int NC, typename T, typename F>
+  __device__ __forceinline__ void q2_gather(const Q2Params &kp, T *buf, F &&value)
+  {
+    using LN       = Q2Lane<DIM>;
+    constexpr int NI = LN::CPW * LN::NP * NC;
+    for (int f = threadIdx.x; f < NI; f += 64)
+      {
+        const int sp = f / NC, c = f - sp * NC;
+        const int s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
+        int       cc[3];
+        T         v = T(0);
+        if (LN::cell(kp, (int64_t)blockIdx.x * LN::CPW + s2, cc))
+          v = value(LN::node(kp, cc, p2) * NC + c);
+        buf[c * 64 + sp] = v;
+      }
+  }
+
+  // store (first colour) or add buf[c * 64 + lane] to
+  // dst[node * os + c * ocs + oo]
+  template <int DIM, int NC, typename T>
+  __device__ __forceinline__ void q2_scatter(const Q2Params &kp, const T *buf, T *dst, int os,
+                                             int ocs, int oo)
+  {
+    using LN       = Q2Lane<DIM>;
+    constexpr int NI = LN::CPW * LN::NP * NC;
+    for (int f = threadIdx.x; f < NI; f += 64)
+      {
+        const int sp = f / NC, c = f - sp * NC;
+        const int s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
+        int       cc[3];
+        if (!LN::cell(kp, (int64_t)blockIdx.x * LN::CPW + s2, cc))
+          continue;
+        T      *d = dst + LN::node(kp, cc, p2) * os + (int64_t)c * ocs + oo;
+        const T r = buf[c * 64 + sp];
+        *d        = LN::first(kp, cc, p2) ? r : *d + r;
+      }
+  }
+
+  // ----------------------------------------- tangent vmult and residual
+  template <int DIM, int G, bool TENS, int OP, typename T>
+  __global__ void __launch_bounds__(64)
+    q2_apply(const Q2Params kp, const T *__restrict__ src, const double *__restrict__ old0,
+             const double *__restrict__ old1, const double *__restrict__ old2,
+             const T *__restrict__ cache, T *__restrict__ dst)
+  {
+    using CL          = CacheLayout<DIM, G, TENS>;
+    constexpr int NC  = G + 2;
+    constexpr int NF  = CL::n_fields;
+    constexpr bool HIST = OP == OP_RES2;
+
+    __shared__ T U[NC * 64];
+    __shared__ T H[HIST ? NC * 64 : 1];
+    __shared__ T W[4][64];
+
+    const Q2Lane<DIM> L(kp);
+    q2_gather<DIM, NC, T>(kp, U, [&](int64_t i) { return src[i]; });
+    if constexpr (HIST)
+      q2_gather<DIM, NC, T>(kp, H, [&](int64_t i) {
+        T h = old0[i] * T(kp.w_hist[0]);
+        if (kp.order > 1)
+          h += old1[i] * T(kp.w_hist[1]);
+        if (kp.order > 2)
+          h += old2[i] * T(kp.w_hist[2]);
+        return h;
+      });
+    __syncthreads();
+
+    const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+    T       v[NC], g[NC][DIM];
+    {
+      T srow[DIM][3], drow[DIM][3];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        {
+          q2_s_row(L.t[d], srow[d]);
+          q2_dq_row(L.t[d], drow[d]);
+        }
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        q2_interp_vg<DIM>(L, U + c * 64, W[0], W[1], W[2], srow, drow, inv_h, v[c], g[c]);
+    }
+
+    T vr[NC], gr[NC][DIM];
+    if constexpr (OP == OP_VMULT)
+      {
+        T        lin[NF];
+        const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          lin[f] = L.valid ? cl[f * 64] : T(0);
+        tangent_qp<DIM, G, TENS>(kp.pp, lin, v, g, vr, gr);
+      }
+    else
+      {
+        T hist[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          hist[c] = T(0);
+        if constexpr (HIST)
+          {
+            T srow[DIM][3];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              q2_s_row(L.t[d], srow[d]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if (c != 1)
+                hist[c] = q2_interp_v<DIM>(L, H + c * 64, W[0], W[1], srow);
+          }
+        residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, v, g, hist, vr, gr);
+      }
+
+    T jxw = T(kp.h_prod);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      jxw *= q2_w<T>(L.t[d]);
+    {
+      T scol[DIM][3], dcol[DIM][3];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        {
+          q2_s_col(L.t[d], scol[d]);
+          q2_dq_col(L.t[d], dcol[d]);
+        }
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        {
+          T gq[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            gq[d] = gr[c][d] * (jxw * inv_h[d]);
+          const T r = q2_integrate<DIM>(L, vr[c] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+          U[c * 64 + L.l] = r;
+        }
+    }
+    __syncthreads();
+    q2_scatter<DIM, NC, T>(kp, U, dst, NC, 1, 0);
+  }
+
+  // ------------------------------------------------ linearisation cache
+  template <int DIM, int G, bool TENS>
+  __global__ void __launch_bounds__(64)
+    q2_fill(const Q2Params kp, const double *__restrict__ lin, double *__restrict__ cache64,
+            float *__restrict__ cache32)
+  {
+    using CL         = CacheLayout<DIM, G, TENS>;
+    constexpr int NC = G + 2;
+    constexpr int NF = CL::n_fields;
+    using T          = double;
+
+    __shared__ T U[NC * 64];
+    __shared__ T W[3][64];
+
+    const Q2Lane<DIM> L(kp);
+    q2_gather<DIM, NC, T>(kp, U, [&](int64_t i) { return lin[i]; });
+    __syncthreads();
+
+    const T inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
+    T       srow[DIM][3], drow[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        q2_s_row(L.t[d], srow[d]);
+        q2_dq_row(L.t[d], drow[d]);
+      }
+    T f[NF];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      {
+        T v, g[DIM];
+        q2_interp_vg<DIM>(L, U + c * 64, W[0], W[1], W[2], srow, drow, inv_h, v, g);
+        if (c == 0)
+          {
+            f[CL::c] = v;
+            if constexpr (TENS)
+#pragma unroll
+              for (int d = 0; d < DIM; ++d)
+                f[CL::grad_c + d] = g[d];
+          }
+        else if (c == 1)
+          {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              f[CL::grad_mu + d] = g[d];
+          }
+        else
+          {
+            f[CL::eta + c - 2] = v;
+            if constexpr (TENS)
+#pragma unroll
+              for (int d = 0; d < DIM; ++d)
+                f[CL::grad_eta + (c - 2) * DIM + d] = g[d];
+          }
+      }
+    const int64_t o = ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+      {
+        cache64[o + q * 64] = f[q];
+        if (cache32)
+          cache32[o + q * 64] = (float)f[q];
+      }
+  }
+
+  // --------------------------------- diagonal / block column by probing
+  // For every local node j of the cell and input component b (all b for the
+  // diagonal, b = column for a block column) the cell's contribution to row
+  // (j, a) of the tangent, a = b for the diagonal and every a for a block
+  // column: sum_q JxW [phi_j (V_a) + grad phi_j . (G_a)] with (V, G) the
+  // quadrature operator applied to the increment e_b phi_j.  The operator is
+  // linear in (value, gradient) of the increment, so it is evaluated once
+  // per direction m of (value, d/dx, ...) with a unit input, and the node
+  // sums over the Gauss points run through LDS.  The result goes to
+  // dst[node * os + a * ocs + oo].
+  template <int DIM, int G, bool TENS>
+  __global__ void __launch_bounds__(64)
+    q2_probe(const Q2Params kp, const double *__restrict__ cache, double *__restrict__ dst,
+             int column, int os, int ocs, int oo)
+  {
+    using CL         = CacheLayout<DIM, G, TENS>;
+    constexpr int NC = G + 2;
+    constexpr int NF = CL::n_fields;
+    using LN         = Q2Lane<DIM>;
+    constexpr int NP = LN::NP;
+    using T          = double;
+
+    __shared__ T X[LN::CPW * NP * NP]; // [cell][node][q]
+    __shared__ T R[NC * 64];
+
+    const LN L(kp);
+    T        lin[NF];
+    {
+      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        lin[f] = L.valid ? cl[f * 64] : T(0);
+    }
+    T jxw = T(kp.h_prod);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      jxw *= q2_w<T>(L.t[d]);
+    // shape values / derivatives of the 1D bases at this lane's Gauss point
+    T sr[DIM][3], er[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        q2_s_row(L.t[d], sr[d]);
+        q2_e_row(L.t[d], er[d]);
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+          er[d][n] *= T(kp.inv_h[d]);
+      }
+    const bool in_cell = L.s < LN::CPW;
+
+    T acc[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+      acc[a] = T(0);
+
+    for (int b = 0; b < NC; ++b)
+      {
+        if (column >= 0 && b != column)
+          continue;
+        for (int m = 0; m <= DIM; ++m)
+          {
+            T v[NC], g[NC][DIM], vr[NC], gr[NC][DIM];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              {
+                v[c] = (c == b && m == 0) ? T(1) : T(0);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  g[c][d] = (c == b && m == d + 1) ? T(1) : T(0);
+              }
+            tangent_qp<DIM, G, TENS>(kp.pp, lin, v, g, vr, gr);
+            for (int a = 0; a < NC; ++a)
+              {
+                if (column < 0 && a != b)
+                  continue;
+                // row a of the output, selected without dynamic register
+                // indexing
+                T va = T(0), ga[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  ga[d] = T(0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                  if (c == a)
+                    {
+                      va = vr[c];
+#pragma unroll
+                      for (int d = 0; d < DIM; ++d)
+                        ga[d] = gr[c][d];
+                    }
+                // X[cell][j][q] = JxW phi_j,m(q) (vr_a phi_j(q) + gr_a . grad phi_j(q))
+                // (a rolled loop with selects: unrolled, the compiler keeps
+                // all 27 shape tuples live and spills)
+                if (in_cell)
+#pragma unroll 1
+                  for (int j = 0; j < NP; ++j)
+                    {
+                      const int jt[3] = {j % 3, (j / 3) % 3, j / 9};
+                      T         sv[DIM], ev[DIM];
+#pragma unroll
+                      for (int d = 0; d < DIM; ++d)
+                        {
+                          sv[d] = sel3(jt[d], sr[d][0], sr[d][1], sr[d][2]);
+                          ev[d] = sel3(jt[d], er[d][0], er[d][1], er[d][2]);
+                        }
+                      T phi = T(1);
+#pragma unroll
+                      for (int d = 0; d < DIM; ++d)
+                        phi *= sv[d];
+                      T w  = va * phi;
+                      T pm = m == 0 ? phi : T(0);
+#pragma unroll
+                      for (int d = 0; d < DIM; ++d)
+                        {
+                          T gd = ev[d];
+#pragma unroll
+                          for (int e = 0; e < DIM; ++e)
+                            if (e != d)
+                              gd *= sv[e];
+                          w += ga[d] * gd;
+                          if (m == d + 1)
+                            pm = gd;
+                        }
+                      X[(L.s * NP + j) * NP + L.p] = jxw * pm * w;
+                    }
+                __syncthreads();
+                if (in_cell)
+                  {
+                    T        s  = T(0);
+                    const T *xr = &X[(L.s * NP + L.p) * NP];
+#pragma unroll
+                    for (int q = 0; q < NP; ++q)
+                      s += xr[q];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                      if (c == a)
+                        acc[c] += s;
+                  }
+                __syncthreads();
+              }
+          }
+      }
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+      R[a * 64 + L.l] = acc[a];
+    __syncthreads();
+    q2_scatter<DIM, NC, T>(kp, R, dst, os, ocs, oo);
+  }
+} // namespace hps

@@ -241,14 +241,17 @@ namespace hps
   }
 
   // stencil features of every node (3D: 27 neighbours, clamped at the
-  // boundary; 2D: 9, the z offsets repeat the plane): per neighbour s
-  // x[s*10 + f] = field f at t_n, x[s*10 + 5 + f] = (f(t_n) - f(t_{n-1}))
-  // * inc_scale with f = c, mu, sum eta^2, eta_{i1}, eta_{i2}; (i1, i2) are
-  // the two largest order parameters at the node at t_n; x[270] = dt *
-  // dt_scale, x[271] = 1.
+  // boundary; 2D: 9, the z offsets repeat the plane).  For the fields
+  // f = c, mu, sum eta^2, eta_{i1}, eta_{i2} of neighbour s:
+  //   x[s*10 + f]     = f(t_n)
+  //   x[s*10 + 5 + f] = (f(t_n) - f(t_{n-1})) / ls
+  // with (i1, i2) the two largest order parameters at the node at t_n and
+  // ls the node's local increment scale, the RMS of the 135 increments plus
+  // a floor; x[270] = dt * dt_scale, x[271] = log10(ls * inc_scale) (the
+  // local scale relative to the global one, 1 / inc_scale).
   __global__ void k_features(FeatParams fp, const double *__restrict__ u_prev,
                              const double *__restrict__ u_cur, uint16_t *__restrict__ X,
-                             int *__restrict__ top2)
+                             int *__restrict__ top2, float *__restrict__ lscale)
   {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= fp.n_nodes)
@@ -281,7 +284,9 @@ namespace hps
     top2[node * 2]     = i1;
     top2[node * 2 + 1] = i2;
     uint16_t *x = X + node * FEAT_STRIDE;
-    int s = 0;
+    double    inc[27][5];
+    double    ss = 0.0;
+    int       s  = 0;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx, ++s)
@@ -302,19 +307,26 @@ namespace hps
 #pragma unroll
             for (int f = 0; f < 5; ++f)
               {
-                x[s * 10 + f]     = f2bf((float)fa[f]);
-                x[s * 10 + 5 + f] = f2bf((float)((fa[f] - fb[f]) * fp.inc_scale));
+                x[s * 10 + f] = f2bf((float)fa[f]);
+                inc[s][f]     = fa[f] - fb[f];
+                ss += inc[s][f] * inc[s][f];
               }
           }
-    x[270] = f2bf((float)(fp.dt * fp.dt_scale));
-    x[271] = f2bf(1.f);
+    const double ls = sqrt(ss / 135.0) + fp.inc_floor;
+    for (int q = 0; q < 27; ++q)
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        x[q * 10 + 5 + f] = f2bf((float)(inc[q][f] / ls));
+    x[270]       = f2bf((float)(fp.dt * fp.dt_scale));
+    x[271]       = f2bf((float)log10(ls * fp.inc_scale));
+    lscale[node] = (float)ls;
   }
 
-  // u_pred = 2 u_n - u_{n-1} + out_scale[k] * y[k] on (c, mu, eta_i1, eta_i2)
+  // u_pred = 2 u_n - u_{n-1} + ls * out_scale[k] * y[k] on (c, mu, eta_i1, eta_i2)
   __global__ void k_predict(int64_t n_nodes, int nc, const double *__restrict__ u_prev,
                             const double *__restrict__ u_cur, const float *__restrict__ Y,
                             const int *__restrict__ top2, const float *__restrict__ out_scale,
-                            double *__restrict__ u_pred)
+                            const float *__restrict__ lscale, double *__restrict__ u_pred)
   {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= n_nodes)
@@ -328,15 +340,15 @@ namespace hps
       {
         if (k == 3 && comp[3] == comp[2])
           break;
-        u_pred[node * nc + comp[k]] += (double)(out_scale[k] * Y[node * 4 + k]);
+        u_pred[node * nc + comp[k]] += (double)lscale[node] * (double)(out_scale[k] * Y[node * 4 + k]);
       }
   }
 
-  // per-row training targets: (u_{n+1} - (2 u_n - u_{n-1}))[comp_k] / out_scale[k]
+  // per-row training targets: (u_{n+1} - (2 u_n - u_{n-1}))[comp_k] / (ls out_scale[k])
   __global__ void k_targets(int64_t n_nodes, int nc, const double *__restrict__ u_prev,
                             const double *__restrict__ u_cur, const double *__restrict__ u_next,
                             const int *__restrict__ top2, const float *__restrict__ out_scale,
-                            float *__restrict__ T)
+                            const float *__restrict__ lscale, float *__restrict__ T)
   {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= n_nodes)
@@ -346,7 +358,9 @@ namespace hps
       {
         const int64_t i = node * nc + comp[k];
         const double  r = u_next[i] - (2.0 * u_cur[i] - u_prev[i]);
-        T[node * 4 + k] = (k == 3 && comp[3] == comp[2]) ? 0.f : (float)(r / out_scale[k]);
+        T[node * 4 + k] = (k == 3 && comp[3] == comp[2])
+                            ? 0.f
+                            : (float)(r / ((double)lscale[node] * (double)out_scale[k]));
       }
   }
 
@@ -402,22 +416,23 @@ namespace hps
                                            Wb, WTb);
   }
   void mlp_features(hipStream_t s, const FeatParams &fp, const double *u_prev,
-                    const double *u_cur, uint16_t *X, int *top2)
+                    const double *u_cur, uint16_t *X, int *top2, float *lscale)
   {
-    k_features<<<(int)((fp.n_nodes + 127) / 128), 128, 0, s>>>(fp, u_prev, u_cur, X, top2);
+    k_features<<<(int)((fp.n_nodes + 127) / 128), 128, 0, s>>>(fp, u_prev, u_cur, X, top2,
+                                                             lscale);
   }
   void mlp_predict(hipStream_t s, int64_t n_nodes, int nc, const double *u_prev,
                    const double *u_cur, const float *Y, const int *top2,
-                   const float *out_scale, double *u_pred)
+                   const float *out_scale, const float *lscale, double *u_pred)
   {
     k_predict<<<(int)((n_nodes + 255) / 256), 256, 0, s>>>(n_nodes, nc, u_prev, u_cur, Y, top2,
-                                                          out_scale, u_pred);
+                                                          out_scale, lscale, u_pred);
   }
   void mlp_targets(hipStream_t s, int64_t n_nodes, int nc, const double *u_prev,
                    const double *u_cur, const double *u_next, const int *top2,
-                   const float *out_scale, float *T)
+                   const float *out_scale, const float *lscale, float *T)
   {
     k_targets<<<(int)((n_nodes + 255) / 256), 256, 0, s>>>(n_nodes, nc, u_prev, u_cur, u_next,
-                                                          top2, out_scale, T);
+                                                          top2, out_scale, lscale, T);
   }
 } // namespace hps

@@ -22,7 +22,7 @@ namespace hps
     int64_t n_nodes;
     int     nn[3];
     int     n_comp;
-    double  dt, dt_scale, inc_scale;
+    double  dt, dt_scale, inc_scale, inc_floor;
   };
 
   void mlp_gemm(hipStream_t, int ta, int tb, int epi, int M, int N, int K, const uint16_t *A,
@@ -36,11 +36,11 @@ namespace hps
                 float lr, float b1, float b2, float eps, float c1, float c2, uint16_t *Wb,
                 uint16_t *WTb);
   void mlp_features(hipStream_t, const FeatParams &, const double *u_prev, const double *u_cur,
-                    uint16_t *X, int *top2);
+                    uint16_t *X, int *top2, float *lscale);
   void mlp_predict(hipStream_t, int64_t n_nodes, int nc, const double *u_prev,
                    const double *u_cur, const float *Y, const int *top2, const float *out_scale,
-                   double *u_pred);
+                   const float *lscale, double *u_pred);
   void mlp_targets(hipStream_t, int64_t n_nodes, int nc, const double *u_prev,
                    const double *u_cur, const double *u_next, const int *top2,
-                   const float *out_scale, float *T);
+                   const float *out_scale, const float *lscale, float *T);
 } // namespace hps

@@ -1,10 +1,11 @@
-// ops.hpp -- launch table of the Q1 cell kernels, one entry per
+// ops.hpp -- launch tables of the Q1 and Q2 cell kernels, one entry per
 // (dim, n_grains, mobility); filled by the inst_*.hip translation units.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "fe_q1.hpp"
+#include "fe_q2.hpp"
 
 namespace hps
 {
@@ -75,6 +76,116 @@ namespace hps
                      &vmult64, &vmult32, &residual, &fill, &probe};
     }
   };
+
+  // ---------------------------------------------------------------- Q2
+  // Every entry runs the 2^dim colour passes in order on one stream.
+  struct Q2Table
+  {
+    int n_fields; // cached linearisation fields per q-point
+    int cpw;      // cells per workgroup
+    void (*vmult64)(const Q2Params &, const double *src, double *dst, const double *cache,
+                    hipStream_t);
+    void (*vmult32)(const Q2Params &, const float *src, float *dst, const float *cache,
+                    hipStream_t);
+    void (*residual)(const Q2Params &, int wtd, const double *src, const double *o0,
+                     const double *o1, const double *o2, double *dst, hipStream_t);
+    void (*fill)(const Q2Params &, const double *lin, double *c64, float *c32, hipStream_t);
+    void (*probe)(const Q2Params &, const double *cache, double *dst, int column, int ostride,
+                  int ocstride, int ooffset, hipStream_t);
+  };
+
+  // parameters of colour `col` (bit d = parity along d); wg: running count
+  // of workgroups of the earlier colours (their cache offset)
+  inline Q2Params q2_colour(Q2Params kp, int dim, int col, int cpw, int64_t &wg)
+  {
+    kp.n_col = 1;
+    for (int d = 0; d < 3; ++d)
+      {
+        kp.par[d]  = d < dim ? (col >> d) & 1 : 0;
+        kp.ncol[d] = d < dim ? (kp.nc[d] - kp.par[d] + 1) / 2 : 1;
+        kp.n_col *= kp.ncol[d];
+      }
+    kp.wg_base = wg;
+    wg += (kp.n_col + cpw - 1) / cpw;
+    return kp;
+  }
+
+  // workgroups of all colours (the cache holds n_fields * 64 entries each)
+  inline int64_t q2_workgroups(const Q2Params &kp, int dim, int cpw)
+  {
+    int64_t wg = 0;
+    for (int col = 0; col < (1 << dim); ++col)
+      (void)q2_colour(kp, dim, col, cpw, wg);
+    return wg;
+  }
+
+  template <int DIM, int G, bool TENS>
+  struct Q2Launch
+  {
+    static constexpr int CPW = Q2Lane<DIM>::CPW;
+
+    template <typename F>
+    static void colours(const Q2Params &kp, F &&launch)
+    {
+      int64_t wg = 0;
+      for (int col = 0; col < (1 << DIM); ++col)
+        {
+          const Q2Params k = q2_colour(kp, DIM, col, CPW, wg);
+          if (k.n_col > 0)
+            launch(k, (unsigned)((k.n_col + CPW - 1) / CPW));
+        }
+    }
+    static void vmult64(const Q2Params &kp, const double *src, double *dst, const double *cache,
+                        hipStream_t s)
+    {
+      colours(kp, [&](const Q2Params &k, unsigned nb) {
+        q2_apply<DIM, G, TENS, OP_VMULT, double>
+          <<<nb, 64, 0, s>>>(k, src, nullptr, nullptr, nullptr, cache, dst);
+      });
+    }
+    static void vmult32(const Q2Params &kp, const float *src, float *dst, const float *cache,
+                        hipStream_t s)
+    {
+      colours(kp, [&](const Q2Params &k, unsigned nb) {
+        q2_apply<DIM, G, TENS, OP_VMULT, float>
+          <<<nb, 64, 0, s>>>(k, src, nullptr, nullptr, nullptr, cache, dst);
+      });
+    }
+    static void residual(const Q2Params &kp, int wtd, const double *src, const double *o0,
+                         const double *o1, const double *o2, double *dst, hipStream_t s)
+    {
+      colours(kp, [&](const Q2Params &k, unsigned nb) {
+        if (wtd == 0)
+          q2_apply<DIM, G, TENS, OP_RES0, double><<<nb, 64, 0, s>>>(k, src, o0, o1, o2, nullptr, dst);
+        else if (wtd == 1)
+          q2_apply<DIM, G, TENS, OP_RES1, double><<<nb, 64, 0, s>>>(k, src, o0, o1, o2, nullptr, dst);
+        else
+          q2_apply<DIM, G, TENS, OP_RES2, double><<<nb, 64, 0, s>>>(k, src, o0, o1, o2, nullptr, dst);
+      });
+    }
+    static void fill(const Q2Params &kp, const double *lin, double *c64, float *c32,
+                     hipStream_t s)
+    {
+      colours(kp, [&](const Q2Params &k, unsigned nb) {
+        q2_fill<DIM, G, TENS><<<nb, 64, 0, s>>>(k, lin, c64, c32);
+      });
+    }
+    static void probe(const Q2Params &kp, const double *cache, double *dst, int column, int os,
+                      int ocs, int oo, hipStream_t s)
+    {
+      colours(kp, [&](const Q2Params &k, unsigned nb) {
+        q2_probe<DIM, G, TENS><<<nb, 64, 0, s>>>(k, cache, dst, column, os, ocs, oo);
+      });
+    }
+    static Q2Table table()
+    {
+      return Q2Table{CacheLayout<DIM, G, TENS>::n_fields, CPW, &vmult64, &vmult32,
+                     &residual, &fill, &probe};
+    }
+  };
+
+  // nullptr when (dim, n_grains, mobility) is not built for Q2
+  const Q2Table *q2_table(int dim, int n_grains, bool tensorial);
 
   // tile extents: 3D 8x4 cells (256 lanes), 2D 64 cells (256 lanes)
   constexpr int TX3 = 8, TY3 = 4;

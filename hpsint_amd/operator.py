@@ -137,14 +137,15 @@ class SinteringOperatorGeneric:
 
     def __init__(self, mesh: CartesianMesh, n_grains: int,
                  free_energy: FreeEnergyParams = None,
-                 mobility: MobilityParams = None, device="cuda"):
+                 mobility: MobilityParams = None, device="cuda", fe_degree=1):
+        """fe_degree 1: FE_Q(1) with QGauss(2); 2: FE_Q(2) with QGauss(3)"""
         free_energy = free_energy or FreeEnergyParams()
         mobility = mobility or MobilityParams()
         self.mesh = mesh
         self.device = torch.device(device)
         p = _ffi.HpsParams()
         p.dim = mesh.dim
-        p.fe_degree = 1
+        p.fe_degree = fe_degree
         for d in range(3):
             p.n_cells[d] = mesh.n_cells[d] if d < mesh.dim else 1
             p.cell_size[d] = mesh.cell_size[d] if d < mesh.dim else 1.0

@@ -131,22 +131,25 @@ class MLP:
 
 
 class LearnedPredictor:
-    """Newton initial guess u_pred = 2 u_n - u_{n-1} + s_n * rel_scale * MLP(features).
+    """Newton initial guess u_pred = 2 u_n - u_{n-1} + ls * out_scale * MLP(features).
 
-    s_n is the RMS of the last increment u_n - u_{n-1} (a library norm); the
-    increment features are divided by s_n and the outputs multiplied by it,
-    so the network sees the same scales at every step of a trajectory."""
+    ls is each node's local increment scale (RMS of the stencil increments
+    of u_n - u_{n-1}, computed by the feature kernel); increment features are
+    divided by it and the outputs multiplied by it, so the network sees the
+    same scales in the quiet bulk and at moving interfaces.  out_scale holds
+    one global constant per output, fixed at training."""
 
-    def __init__(self, op, mlp: MLP = None, dt_scale=1.0, rel_scale=None):
+    def __init__(self, op, mlp: MLP = None, dt_scale=1.0, out_scale=None):
         self.op = op
         self.mlp = mlp or MLP(op)
         self.dt_scale = dt_scale
         dev = self.mlp.W[0].device
-        self.rel_scale = (list(rel_scale) if rel_scale is not None else [1.0] * N_OUT)
-        self.out_scale = torch.ones(N_OUT, dtype=torch.float32, device=dev)
+        self.out_scale = torch.tensor(list(out_scale) if out_scale is not None else [1.0] * N_OUT,
+                                      dtype=torch.float32, device=dev)
         n = op.n_nodes
         self.X = torch.empty(n, N_IN, dtype=torch.bfloat16, device=dev)
         self.top2 = torch.empty(n, 2, dtype=torch.int32, device=dev)
+        self.lscale = torch.empty(n, dtype=torch.float32, device=dev)
         self.diff = torch.empty(op.n_dofs, dtype=torch.float64, device=dev)
         self.use_network = True
         self.inc_scale = 1.0
@@ -159,16 +162,11 @@ class LearnedPredictor:
         check(lib().hps_axpy(h, n, -1.0, _p(u_prev), _p(self.diff)))
         return max(self.op.norm(self.diff) / math.sqrt(n), 1e-300)
 
-    def _set_scales(self, u_prev, u_cur):
-        s = self.step_scale(u_prev, u_cur)
-        self.inc_scale = 1.0 / s
-        self.out_scale.copy_(torch.tensor([s * r for r in self.rel_scale], dtype=torch.float32))
-        return s
-
     def features(self, u_prev, u_cur, dt):
-        self._set_scales(u_prev, u_cur)
+        self.inc_scale = 1.0 / self.step_scale(u_prev, u_cur)
         check(lib().hps_mlp_features(self.op.handle, _p(u_prev), _p(u_cur), dt, self.dt_scale,
-                                     self.inc_scale, _p(self.X), _p(self.top2)))
+                                     self.inc_scale, _p(self.X), _p(self.top2),
+                                     _p(self.lscale)))
         return self.X, self.top2
 
     def predict(self, u_prev, u_cur, dt, out):
@@ -177,14 +175,15 @@ class LearnedPredictor:
             X, _ = self.features(u_prev, u_cur, dt)
             Y, _ = self.mlp.forward(X, self.op.n_nodes)
         check(lib().hps_mlp_predict(self.op.handle, _p(u_prev), _p(u_cur), _p(Y),
-                                    _p(self.top2), _p(self.out_scale), _p(out)))
+                                    _p(self.top2), _p(self.out_scale), _p(self.lscale),
+                                    _p(out)))
         return out
 
     def targets(self, u_prev, u_cur, u_next, T):
-        """(u_next - (2 u_cur - u_prev)) / out_scale on the 4 outputs; call
-        after features() of the same (u_prev, u_cur)"""
+        """(u_next - (2 u_cur - u_prev)) / (ls * out_scale) on the 4 outputs;
+        call after features() of the same (u_prev, u_cur)"""
         check(lib().hps_mlp_targets(self.op.handle, _p(u_prev), _p(u_cur), _p(u_next),
-                                    _p(self.top2), _p(self.out_scale), _p(T)))
+                                    _p(self.top2), _p(self.out_scale), _p(self.lscale), _p(T)))
         return T
 
     def __call__(self, x, states, dt):

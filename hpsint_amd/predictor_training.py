@@ -31,31 +31,14 @@ def collect_trajectory(op, u0, n_steps, dt, nl_rel_tol=1e-8, l_rel_tol=1e-2,
     return traj
 
 
-def _rms(x):
-    return float(torch.sqrt(torch.mean(x.double() ** 2)))
-
-
-def relative_scales(op, traj):
-    """RMS over the trajectory of the second difference divided by the step
-    scale s_n = RMS(u_n - u_{n-1}), per output group (c, mu, eta, eta)"""
-    nc = op.n_components()
-    acc = [0.0, 0.0, 0.0]
-    for i in range(1, len(traj) - 1):
-        s = max(_rms(traj[i] - traj[i - 1]), 1e-300)
-        sec = ((traj[i + 1] - 2 * traj[i] + traj[i - 1]) / s).view(-1, nc)
-        acc[0] += _rms(sec[:, 0]) ** 2
-        acc[1] += _rms(sec[:, 1]) ** 2
-        acc[2] += _rms(sec[:, 2:]) ** 2
-    m = max(len(traj) - 2, 1)
-    c, mu, eta = (max(math.sqrt(a / m), 1e-12) for a in acc)
-    return [c, mu, eta, eta]
-
-
 def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
+    """rows of (features, targets) from every triple of traj; sets
+    pred.out_scale to the RMS of each target column and normalises by it"""
     op = pred.op
     n = op.n_nodes
     dev = pred.X.device
     g = torch.Generator(device="cpu").manual_seed(seed)
+    pred.out_scale.fill_(1.0)
     Xs, Ts = [], []
     T = torch.empty(n, N_OUT, dtype=torch.float32, device=dev)
     for i in range(1, len(traj) - 1):
@@ -68,7 +51,11 @@ def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
         idx = torch.cat([top, rnd])
         Xs.append(X.index_select(0, idx))
         Ts.append(T.index_select(0, idx))
-    return torch.cat(Xs), torch.cat(Ts)
+    X, T = torch.cat(Xs), torch.cat(Ts)
+    scale = T.double().pow(2).mean(dim=0).sqrt().clamp_min(1e-12).float()
+    T /= scale
+    pred.out_scale.copy_(scale)
+    return X, T
 
 
 def fit_predictor(op, traj, dt, iterations=1000, rows_per_step=32768, batch=16384,
@@ -77,7 +64,7 @@ def fit_predictor(op, traj, dt, iterations=1000, rows_per_step=32768, batch=1638
     relaxation of the initial condition and are left out)"""
     traj = traj[skip:]
     mlp = MLP(op, seed=seed)
-    pred = LearnedPredictor(op, mlp, dt_scale=1.0 / dt, rel_scale=relative_scales(op, traj))
+    pred = LearnedPredictor(op, mlp, dt_scale=1.0 / dt)
     X, T = build_dataset(pred, traj, dt, rows_per_step, seed)
     m = X.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed + 1)

@@ -66,7 +66,8 @@ typedef struct hps_operator_s *hps_operator;
 typedef struct
 {
   int    dim;           /* 2 or 3 */
-  int    fe_degree;     /* 1 (FE_Q(1), QGauss(2)) */
+  int    fe_degree;     /* 1 (FE_Q(1), QGauss(2)) or 2 (FE_Q(2), QGauss(3));
+                           nodes per direction: fe_degree * n_cells + 1 */
   int    n_cells[3];    /* cells per direction */
   double cell_size[3];  /* h per direction */
   int    n_components;  /* n_grains + 2 */
@@ -268,17 +269,22 @@ int hps_mlp_loss(hps_operator op, int64_t n, const float *Y, const float *T, voi
 int hps_mlp_adam(hps_operator op, int n_out, int n_in, float *W, float *m1, float *m2,
                  const float *grad, float lr, float beta1, float beta2, float eps, int step,
                  void *W_bf16);
-/* per node: 272 bf16 stencil features and the node's top-2 order
- * parameters at t_n (top2: 2 ints per node) */
+/* per node: 272 bf16 stencil features, the node's top-2 order parameters
+ * at t_n (top2: 2 ints per node) and its local increment scale ls (lscale:
+ * 1 float per node; increments are divided by it).  inc_scale = 1 / (global
+ * increment scale); the floor of ls is 1e-3 / inc_scale. */
 int hps_mlp_features(hps_operator op, const double *u_prev, const double *u_cur, double dt,
-                     double dt_scale, double inc_scale, void *X, int *top2);
-/* u_pred = 2 u_cur - u_prev, plus out_scale[k] * Y[node*4+k] on
+                     double dt_scale, double inc_scale, void *X, int *top2, float *lscale);
+/* u_pred = 2 u_cur - u_prev, plus ls * out_scale[k] * Y[node*4+k] on
  * (c, mu, eta_top1, eta_top2) when Y != NULL */
 int hps_mlp_predict(hps_operator op, const double *u_prev, const double *u_cur,
-                    const float *Y, const int *top2, const float *out_scale, double *u_pred);
-/* training targets: (u_next - (2 u_cur - u_prev)) / out_scale on the 4 outputs */
+                    const float *Y, const int *top2, const float *out_scale,
+                    const float *lscale, double *u_pred);
+/* training targets: (u_next - (2 u_cur - u_prev)) / (ls * out_scale) on the
+ * 4 outputs */
 int hps_mlp_targets(hps_operator op, const double *u_prev, const double *u_cur,
-                    const double *u_next, const int *top2, const float *out_scale, float *T);
+                    const double *u_next, const int *top2, const float *out_scale,
+                    const float *lscale, float *T);
 
 #ifdef __cplusplus
 }
