@@ -42,7 +42,71 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=15,
                     help="plain-solver steps that train the predictor")
     ap.add_argument("--train-iterations", type=int, default=1500)
+    ap.add_argument("--q2-cells", type=int, default=96,
+                    help="cells per direction of the Q2 tensorial workload (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run hps_copy of this many doubles 5 times (calibrates "
+                         "the HBM byte counters against a copy of known size) and exit")
     return ap.parse_args()
+
+
+def copy_probe(n):
+    """5 plain copies of n doubles (8 n bytes read + 8 n written each)"""
+    import ctypes
+    import torch
+    from hpsint_amd import CartesianMesh, SinteringOperatorGeneric
+    from hpsint_amd._ffi import check, lib
+    op = SinteringOperatorGeneric(CartesianMesh((4, 4, 4), (0, 0, 0), (4.0, 4.0, 4.0)), 1)
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        check(lib().hps_copy(op.handle, n, ctypes.c_void_p(y.data_ptr()),
+                             ctypes.c_void_p(x.data_ptr())))
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n}))
+
+
+def q2_vmult(args, timed):
+    """FP64 and FP32 tangent vmult of the Q2 workload: random sequential
+    packing of 64 spheres (lognormal radii, mean 8, sigma 0.2) in a 160^3
+    box, Q2 on a uniform q2_cells^3 mesh (3^3 Gauss points), tensorial
+    mobility, 10 order parameters (12 components)"""
+    import torch
+    from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,
+                            SinteringOperatorGeneric, packing)
+    n = args.q2_cells
+    pk = packing.packing_cfg3(seed=0)
+    G = 10
+    if pk.n_order_params > G:
+        raise RuntimeError("packing needs %d order parameters" % pk.n_order_params)
+    mesh = CartesianMesh((n, n, n), pk.lower, pk.upper)
+    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(),
+                                  MobilityParams(tensorial=True), fe_degree=2)
+    op.set_time_weights(1e3, (-1e3,))
+    # Q2 nodes of an n^3 mesh = Q1 nodes of a (2n)^3 mesh
+    u = torch.from_numpy(packing.initial_fields(pk, (2 * n,) * 3, n_order_params=G)).cuda()
+    op.fill_quadrature_point_values(u)
+    src = torch.from_numpy(np.random.default_rng(1234).uniform(-1, 1, op.n_dofs)).cuda()
+    dst = op.initialize_dof_vector()
+    for _ in range(3):
+        op.vmult(dst, src)
+    k = max(5, args.steps // 5)
+    t64 = timed(lambda: op.vmult(dst, src), k)
+    src32, dst32 = src.float(), op.initialize_dof_vector(torch.float32)
+    for _ in range(3):
+        op.vmult(dst32, src32)
+    t32 = timed(lambda: op.vmult(dst32, src32), k)
+    nf = 1 + G + (2 + G) * 3
+    cells = n ** 3
+    bytes64 = 8 * (2 * op.n_dofs + cells * 27 * nf)
+    out = {"workload": "Q2 %d^3, 64 spheres, tensorial mobility, %d components, %d DoF"
+                       % (n, G + 2, op.n_dofs),
+           "n_dofs": op.n_dofs, "vmult_f64_gdofs": op.n_dofs / t64 / 1e9,
+           "vmult_f64_ms": t64 * 1e3, "vmult_f32_gdofs": op.n_dofs / t32 / 1e9,
+           "vmult_f64_hbm_frac": bytes64 / t64 / 1e9 / HBM_PEAK_GBS}
+    del op, u, src, dst, src32, dst32
+    torch.cuda.empty_cache()
+    return out
 
 
 def timestep_comparison(op, u0, dt, args):
@@ -117,6 +181,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     import torch
     from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,
@@ -223,6 +290,8 @@ def main():
                          "oracle (direct per-cell quadrature, OpenMP over colour classes; "
                          "CPU restatement, not deal.II) on %s" % (reps, _cpu_model())}
 
+    q2 = q2_vmult(args, timed) if args.q2_cells > 0 else None
+
     steps = None
     if args.timesteps > 0:
         steps = timestep_comparison(op, u, dt, args)
@@ -260,6 +329,7 @@ def main():
         "residual_hbm_frac": res_achieved / HBM_PEAK_GBS,
         "vmult_f32_gdofs": n_dofs / t_vm32 / 1e9,
         "timesteps": steps,
+        "q2_tensorial": q2,
     }
     print(json.dumps(out))
 

@@ -100,38 +100,47 @@ namespace hps
     static constexpr T sb = T(0.21132486540518711774542560974902);
   };
 
-  // values and gradients at the lane's q-point from the lane's nodal value
+  // signed scale of the lane's derivative along d: +1/h_d if the lane's
+  // corner bit d is set, -1/h_d otherwise (times an optional factor)
   template <int DIM, typename T>
-  __device__ __forceinline__ void interp_vg(T u, int corner, const T *inv_h, T &val,
-                                            T *grad)
+  __device__ __forceinline__ void signed_scales(int corner, const T *inv_h, T f, T *out)
+  {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      out[d] = ((corner >> d) & 1) ? f * inv_h[d] : -(f * inv_h[d]);
+  }
+
+  // values and gradients at the lane's q-point from the lane's nodal value;
+  // gsc: signed_scales(corner, inv_h, 1).  The derivative's sign is applied
+  // with the scale at the end (exact: partners along the other directions
+  // share it).
+  template <int DIM, typename T>
+  __device__ __forceinline__ void interp_vg(T u, const T *gsc, T &val, T *grad)
   {
     const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
-    const T s0 = (corner & 1) ? T(1) : T(-1);
-    const T s1 = (corner & 2) ? T(1) : T(-1);
     const T p  = xshfl<1>(u);
     const T A  = sa * u + sb * p;
-    const T Dx = s0 * (u - p);
+    const T Dx = u - p;
     const T pA = xshfl<2>(A);
     const T pD = xshfl<2>(Dx);
     if constexpr (DIM == 2)
       {
         val     = sa * A + sb * pA;
-        grad[1] = s1 * (A - pA) * inv_h[1];
-        grad[0] = (sa * Dx + sb * pD) * inv_h[0];
+        grad[1] = (A - pA) * gsc[1];
+        grad[0] = (sa * Dx + sb * pD) * gsc[0];
       }
     else
       {
-        const T s2   = (corner & 4) ? T(1) : T(-1);
         const T AA   = sa * A + sb * pA;
-        const T Ay   = s1 * (A - pA);
+        const T Ay   = A - pA;
         const T DxA  = sa * Dx + sb * pD;
         const T pAA  = xshfl<4>(AA);
         const T pAy  = xshfl<4>(Ay);
         const T pDxA = xshfl<4>(DxA);
         val          = sa * AA + sb * pAA;
-        grad[2]      = s2 * (AA - pAA) * inv_h[2];
-        grad[1]      = (sa * Ay + sb * pAy) * inv_h[1];
-        grad[0]      = (sa * DxA + sb * pDxA) * inv_h[0];
+        grad[2]      = (AA - pAA) * gsc[2];
+        grad[1]      = (sa * Ay + sb * pAy) * gsc[1];
+        grad[0]      = (sa * DxA + sb * pDxA) * gsc[0];
       }
   }
 
@@ -146,43 +155,42 @@ namespace hps
     return a;
   }
 
-  // transpose of interp_vg: nodal contribution of (v, g) given at q-points
-  // (g already scaled by JxW/h)
+  // transpose of interp_vg: nodal contribution of (v, g) given at q-points,
+  // g already multiplied by signed_scales(corner, inv_h, JxW): the partner
+  // across direction d holds the opposite sign, so s_d (g + g_partner) is
+  // the difference of the signed values
   template <int DIM, typename T>
-  __device__ __forceinline__ T integrate_vg(T v, const T *g, int corner)
+  __device__ __forceinline__ T integrate_vg(T v, const T *g)
   {
     const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
-    const T s0 = (corner & 1) ? T(1) : T(-1);
-    const T s1 = (corner & 2) ? T(1) : T(-1);
     if constexpr (DIM == 2)
       {
         const T pv  = xshfl<2>(v);
         const T pgy = xshfl<2>(g[1]);
         const T pgx = xshfl<2>(g[0]);
-        const T U1  = sa * v + sb * pv + s1 * (g[1] + pgy);
+        const T U1  = sa * v + (sb * pv + (g[1] - pgy));
         const T Ux  = sa * g[0] + sb * pgx;
         const T pU1 = xshfl<1>(U1);
         const T pUx = xshfl<1>(Ux);
-        return sa * U1 + sb * pU1 + s0 * (Ux + pUx);
+        return sa * U1 + (sb * pU1 + (Ux - pUx));
       }
     else
       {
-        const T s2  = (corner & 4) ? T(1) : T(-1);
         const T pv  = xshfl<4>(v);
         const T pgz = xshfl<4>(g[2]);
         const T pgy = xshfl<4>(g[1]);
         const T pgx = xshfl<4>(g[0]);
-        const T T1  = sa * v + sb * pv + s2 * (g[2] + pgz);
+        const T T1  = sa * v + (sb * pv + (g[2] - pgz));
         const T Ty  = sa * g[1] + sb * pgy;
         const T Tx  = sa * g[0] + sb * pgx;
         const T pT1 = xshfl<2>(T1);
         const T pTy = xshfl<2>(Ty);
         const T pTx = xshfl<2>(Tx);
-        const T U1  = sa * T1 + sb * pT1 + s1 * (Ty + pTy);
+        const T U1  = sa * T1 + (sb * pT1 + (Ty - pTy));
         const T Ux  = sa * Tx + sb * pTx;
         const T pU1 = xshfl<1>(U1);
         const T pUx = xshfl<1>(Ux);
-        return sa * U1 + sb * pU1 + s0 * (Ux + pUx);
+        return sa * U1 + (sb * pU1 + (Ux - pUx));
       }
   }
 
@@ -300,7 +308,10 @@ namespace hps
     const TL t(kp);
     const int tid = t.tid;
 
-    T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+    const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+    T       gsc[DIM], gsj[DIM];
+    signed_scales<DIM>(t.corner, inv_h, T(1), gsc);
+    signed_scales<DIM>(t.corner, inv_h, T(kp.jxw), gsj);
     const T jxw = T(kp.jxw);
 
     T regs[NPL][ITEMS];
@@ -373,7 +384,7 @@ namespace hps
         T        val[NC], grad[NC][DIM];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-          interp_vg<DIM>(pl[c], t.corner, inv_h, val[c], grad[c]);
+          interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
 
         T vr[NC], gr[NC][DIM];
         if constexpr (OP == OP_VMULT)
@@ -405,8 +416,8 @@ namespace hps
             T gs[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-              gs[d] = gr[c][d] * (jxw * inv_h[d]);
-            sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs, t.corner);
+              gs[d] = gr[c][d] * gsj[d];
+            sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
           }
         __syncthreads();
 
@@ -477,6 +488,8 @@ namespace hps
     const TL          t(kp);
     const int         tid      = t.tid;
     const double      inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
+    double            gsc[DIM];
+    signed_scales<DIM>(t.corner, inv_h, 1.0, gsc);
 
     double regs[ITEMS];
     auto   load_plane = [&](int k) {
@@ -524,7 +537,7 @@ namespace hps
             for (int i = 0; i < G; ++i)
               f[CL::eta + i] = interp_v<DIM>(pl[2 + i]);
             double v, g[DIM];
-            interp_vg<DIM>(pl[1], t.corner, inv_h, v, g);
+            interp_vg<DIM>(pl[1], gsc, v, g);
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
               f[CL::grad_mu + d] = g[d];
@@ -532,19 +545,19 @@ namespace hps
         else
           {
             double v, g[DIM];
-            interp_vg<DIM>(pl[0], t.corner, inv_h, v, g);
+            interp_vg<DIM>(pl[0], gsc, v, g);
             f[CL::c] = v;
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
               f[CL::grad_c + d] = g[d];
-            interp_vg<DIM>(pl[1], t.corner, inv_h, v, g);
+            interp_vg<DIM>(pl[1], gsc, v, g);
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
               f[CL::grad_mu + d] = g[d];
 #pragma unroll
             for (int i = 0; i < G; ++i)
               {
-                interp_vg<DIM>(pl[2 + i], t.corner, inv_h, v, g);
+                interp_vg<DIM>(pl[2 + i], gsc, v, g);
                 f[CL::eta + i] = v;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)

@@ -160,6 +160,15 @@ def random_packing(n, mean, sigma, box, seed=0, max_tries=200000, overlap=0.5):
     return p
 
 
+def packing_cfg3(seed=0):
+    """random sequential packing of 64 spheres, lognormal radii (mean 8,
+    sigma 0.2) in a 160^3 box; the interaction buffer (2.075 max radius)
+    spreads the grains over 10 order parameters for seed 0"""
+    p = random_packing(64, 8.0, 0.2, 160.0, seed=seed)
+    p.colors = greedy_coloring(interaction_graph(p, radius_buffer_ratio=2.075))
+    return p
+
+
 def initial_fields(p: Packing, n_cells, w=1.0, n_order_params=None, dtype=np.float64):
     """interleaved nodal state (node-major, x fastest): c, mu, eta_0.."""
     dim = p.dim
