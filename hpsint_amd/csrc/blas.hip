@@ -48,80 +48,180 @@ namespace hps
       partial[blockIdx.x] = s;
   }
 
-  // sums nb partial rows of k values each: out[j] = sum_b partial[b*k + j]
+  // sums nb partial rows of k values each: out[j] = sum_b partial[b*k + j];
+  // wave w handles j = w, w + 4, ... (fixed order: deterministic)
   __global__ void __launch_bounds__(RB)
     k_finish(int nb, int k, const double *__restrict__ partial, double *__restrict__ out,
              int do_sqrt)
   {
-    __shared__ double sh[RB / 64];
-    for (int j = 0; j < k; ++j)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int j = w; j < k; j += RB / 64)
       {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += RB)
+        for (int b = lane; b < nb; b += 64)
           s += partial[(int64_t)b * k + j];
-        s = block_reduce(s, sh);
-        if (threadIdx.x == 0)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+          s += __shfl_down(s, o, 64);
+        if (lane == 0)
           out[j] = do_sqrt ? sqrt(s) : s;
       }
   }
 
-  // h[j] = <V_j, w>, j < k (k <= MAXK)
-  template <int KK>
-  __global__ void __launch_bounds__(RB)
-    k_multi_dot(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ w,
-                double *__restrict__ partial)
+  // K values reduced over the block in one pass: per-wave shuffle trees,
+  // then thread j < K sums the waves' partials in wave order
+  template <int K>
+  __device__ __forceinline__ void block_reduce_k(double *acc, double (*sh)[K],
+                                                 double *__restrict__ out)
   {
-    __shared__ double sh[RB / 64];
-    double            acc[KK];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < KK; ++j)
+    for (int j = 0; j < K; ++j)
+      {
+        double v = acc[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+          v += __shfl_down(v, o, 64);
+        if (lane == 0)
+          sh[w][j] = v;
+      }
+    __syncthreads();
+    if (threadIdx.x < K)
+      {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < RB / 64; ++i)
+          t += sh[i][threadIdx.x];
+        out[threadIdx.x] = t;
+      }
+  }
+
+  // grid-stride steps unrolled so that a thread has >= 8 loads in flight
+  template <int K>
+  constexpr int unroll_for()
+  {
+    return K >= 8 ? 1 : (K >= 4 ? 2 : 4);
+  }
+
+  // h[j] = <V_j, w>, j < K; rows of V have length n; storage T (FP64 or
+  // FP32), FP64 accumulation.  K is exact, so every row load of a step is
+  // issued before the first is consumed.
+  // (partial row of block b: partial[b * pstride + pcol + j])
+  template <int K, typename T>
+  __global__ void __launch_bounds__(RB)
+    k_mdot(int64_t n, const T *__restrict__ V, const T *__restrict__ w,
+           double *__restrict__ partial, int pstride, int pcol)
+  {
+    constexpr int     U = unroll_for<K>();
+    __shared__ double sh[RB / 64][K];
+    double            acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
       acc[j] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB)
+    const int64_t st = (int64_t)gridDim.x * RB;
+    int64_t       i  = (int64_t)blockIdx.x * RB + threadIdx.x;
+    for (; i + (U - 1) * st < n; i += U * st)
+      {
+        T wv[U], vv[U][K];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          {
+            wv[u] = w[i + u * st];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+              vv[u][j] = V[(int64_t)j * n + i + u * st];
+          }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            acc[j] += (double)vv[u][j] * (double)wv[u];
+      }
+    for (; i < n; i += st)
       {
         const double wi = w[i];
 #pragma unroll
-        for (int j = 0; j < KK; ++j)
-          if (j < k)
-            acc[j] += V[(int64_t)j * n + i] * wi;
+        for (int j = 0; j < K; ++j)
+          acc[j] += (double)V[(int64_t)j * n + i] * wi;
       }
-#pragma unroll
-    for (int j = 0; j < KK; ++j)
-      if (j < k)
-        {
-          const double s = block_reduce(acc[j], sh);
-          if (threadIdx.x == 0)
-            partial[(int64_t)blockIdx.x * k + j] = s;
-        }
+    block_reduce_k<K>(acc, sh, partial + (int64_t)blockIdx.x * pstride + pcol);
   }
 
-  // w -= sum_j h[j] V_j ; partial norms of the updated w
-  template <int KK>
+  // w -= sum_j h[j] V_j (FP64 arithmetic, stored as T); partial sums of
+  // squares of the stored w
+  template <int K, typename T>
   __global__ void __launch_bounds__(RB)
-    k_multi_axpy_norm(int64_t n, int k, const double *__restrict__ V,
-                      const double *__restrict__ h, double *__restrict__ w,
-                      double *__restrict__ partial)
+    k_maxpy_norm(int64_t n, const T *__restrict__ V, const double *__restrict__ h,
+                 T *__restrict__ w, double *__restrict__ partial)
   {
-    __shared__ double sh[RB / 64];
-    double            hk[KK];
+    constexpr int     U = unroll_for<K>();
+    __shared__ double sh[RB / 64][1];
+    double            hk[K];
 #pragma unroll
-    for (int j = 0; j < KK; ++j)
-      hk[j] = j < k ? h[j] : 0.0;
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB)
+    for (int j = 0; j < K; ++j)
+      hk[j] = h[j];
+    double        s  = 0.0;
+    const int64_t st = (int64_t)gridDim.x * RB;
+    int64_t       i  = (int64_t)blockIdx.x * RB + threadIdx.x;
+    for (; i + (U - 1) * st < n; i += U * st)
+      {
+        T wv[U], vv[U][K];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          {
+            wv[u] = w[i + u * st];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+              vv[u][j] = V[(int64_t)j * n + i + u * st];
+          }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          {
+            double wi = wv[u];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+              wi -= hk[j] * (double)vv[u][j];
+            const T wt      = (T)wi;
+            w[i + u * st]   = wt;
+            s += (double)wt * (double)wt;
+          }
+      }
+    for (; i < n; i += st)
       {
         double wi = w[i];
 #pragma unroll
-        for (int j = 0; j < KK; ++j)
-          if (j < k)
-            wi -= hk[j] * V[(int64_t)j * n + i];
-        w[i] = wi;
-        s += wi * wi;
+        for (int j = 0; j < K; ++j)
+          wi -= hk[j] * (double)V[(int64_t)j * n + i];
+        const T wt = (T)wi;
+        w[i]       = wt;
+        s += (double)wt * (double)wt;
       }
-    s = block_reduce(s, sh);
-    if (threadIdx.x == 0)
-      partial[blockIdx.x] = s;
+    double acc[1] = {s};
+    block_reduce_k<1>(acc, sh, partial + blockIdx.x);
+  }
+
+  // y (FP64) += sum_j a_j X_j (storage T), j < K
+  template <int K, typename T>
+  __global__ void k_axpy_multi(int64_t n, const double *__restrict__ a, const T *__restrict__ X,
+                               double *__restrict__ y)
+  {
+    double ak[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      ak[j] = a[j];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      {
+        T xv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          xv[j] = X[(int64_t)j * n + i];
+        double s = y[i];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          s += ak[j] * (double)xv[j];
+        y[i] = s;
+      }
   }
 
   // per-component sums of squares over an interleaved vector
@@ -211,24 +311,31 @@ namespace hps
   // FullMatrix::gauss_jordan in InverseComponentBlockDiagonalMatrix::
   // do_update, I/lac/preconditioners.h:288-308)
   template <int NC>
-  __global__ void k_invert_blocks(int64_t n_nodes, const double *__restrict__ blocks,
+  __global__ void __launch_bounds__(64)
+    k_invert_blocks(int64_t n_nodes, const double *__restrict__ blocks,
                                   double *__restrict__ inv64, float *__restrict__ inv32,
                                   int *__restrict__ singular)
   {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= n_nodes)
       return;
+    // fully unrolled; the pivot row swap is done by selects, so both
+    // matrices stay in registers
     double a[NC][NC], b[NC][NC];
+#pragma unroll
     for (int i = 0; i < NC; ++i)
+#pragma unroll
       for (int j = 0; j < NC; ++j)
         {
           a[i][j] = blocks[(node * NC + i) * NC + j];
           b[i][j] = i == j ? 1.0 : 0.0;
         }
+#pragma unroll
     for (int j = 0; j < NC; ++j)
       {
         int    r   = j;
         double max = fabs(a[j][j]);
+#pragma unroll
         for (int i = j + 1; i < NC; ++i)
           if (fabs(a[i][j]) > max)
             {
@@ -240,27 +347,34 @@ namespace hps
             atomicAdd(singular, 1);
             return;
           }
-        if (r != j)
-          for (int k = 0; k < NC; ++k)
-            {
-              double t = a[j][k];
-              a[j][k]  = a[r][k];
-              a[r][k]  = t;
-              t        = b[j][k];
-              b[j][k]  = b[r][k];
-              b[r][k]  = t;
-            }
+#pragma unroll
+        for (int i = j + 1; i < NC; ++i)
+          {
+            const bool sw = i == r;
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+              {
+                const double aj = a[j][k], ai = a[i][k], bj = b[j][k], bi = b[i][k];
+                a[j][k]         = sw ? ai : aj;
+                a[i][k]         = sw ? aj : ai;
+                b[j][k]         = sw ? bi : bj;
+                b[i][k]         = sw ? bj : bi;
+              }
+          }
         const double hr = 1.0 / a[j][j];
+#pragma unroll
         for (int k = 0; k < NC; ++k)
           {
             a[j][k] *= hr;
             b[j][k] *= hr;
           }
+#pragma unroll
         for (int i = 0; i < NC; ++i)
           {
             if (i == j)
               continue;
             const double f = a[i][j];
+#pragma unroll
             for (int k = 0; k < NC; ++k)
               {
                 a[i][k] -= f * a[j][k];
@@ -346,28 +460,60 @@ namespace hps
     k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, nc, partial, out, 1);
   }
 
+#define HPS_K_SWITCH(k, CALL)                                                  \
+  switch (k)                                                                   \
+    {                                                                          \
+      case 1: CALL(1); break;   case 2: CALL(2); break;   case 3: CALL(3); break;   \
+      case 4: CALL(4); break;   case 5: CALL(5); break;   case 6: CALL(6); break;   \
+      case 7: CALL(7); break;   case 8: CALL(8); break;   case 9: CALL(9); break;   \
+      case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break; \
+      case 13: CALL(13); break; case 14: CALL(14); break; case 15: CALL(15); break; \
+      case 16: CALL(16); break; case 17: CALL(17); break; case 18: CALL(18); break; \
+      case 19: CALL(19); break; case 20: CALL(20); break; case 21: CALL(21); break; \
+      case 22: CALL(22); break; case 23: CALL(23); break; case 24: CALL(24); break; \
+      case 25: CALL(25); break; case 26: CALL(26); break; case 27: CALL(27); break; \
+      case 28: CALL(28); break; case 29: CALL(29); break; case 30: CALL(30); break; \
+      case 31: CALL(31); break; case 32: CALL(32); break;                       \
+      default: break;                                                          \
+    }
+
+  // rows in passes of at most 16 (beyond that the compiler serialises the
+  // row loads of one step and the kernel drops to half bandwidth)
+  template <typename T>
+  static void multi_dot_any(hipStream_t s, int64_t n, int k, const T *V, const T *w,
+                            double *partial, double *out)
+  {
+    for (int j0 = 0; j0 < k; j0 += 16)
+      {
+        const int kk = k - j0 < 16 ? k - j0 : 16;
+        const T  *Vj = V + (int64_t)j0 * n;
+#define HPS_CALL(K) k_mdot<K, T><<<RED_BLOCKS, RB, 0, s>>>(n, Vj, w, partial, k, j0)
+        HPS_K_SWITCH(kk, HPS_CALL)
+#undef HPS_CALL
+      }
+    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, k, partial, out, 0);
+  }
+
+  template <typename T>
+  static void multi_axpy_norm_any(hipStream_t s, int64_t n, int k, const T *V,
+                                  const double *h, T *w, double *partial, double *out)
+  {
+#define HPS_CALL(K) k_maxpy_norm<K, T><<<RED_BLOCKS, RB, 0, s>>>(n, V, h, w, partial)
+    HPS_K_SWITCH(k, HPS_CALL)
+#undef HPS_CALL
+    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+  }
+
   void blas_multi_dot(hipStream_t s, int64_t n, int k, const double *V, const double *w,
                       double *partial, double *out)
   {
-    if (k <= 8)
-      k_multi_dot<8><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
-    else if (k <= 16)
-      k_multi_dot<16><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
-    else
-      k_multi_dot<MAXK><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, k, partial, out, 0);
+    multi_dot_any<double>(s, n, k, V, w, partial, out);
   }
 
   void blas_multi_axpy_norm(hipStream_t s, int64_t n, int k, const double *V,
                             const double *h, double *w, double *partial, double *out)
   {
-    if (k <= 8)
-      k_multi_axpy_norm<8><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
-    else if (k <= 16)
-      k_multi_axpy_norm<16><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
-    else
-      k_multi_axpy_norm<MAXK><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+    multi_axpy_norm_any<double>(s, n, k, V, h, w, partial, out);
   }
 
   void blas_axpy(hipStream_t s, int64_t n, double a, const double *x, double *y)
@@ -445,8 +591,8 @@ namespace hps
   bool invert_blocks(hipStream_t s, int64_t n_nodes, int nc, const double *blocks,
                      double *inv64, float *inv32, int *singular)
   {
-    const int g = (int)((n_nodes + 127) / 128);
-#define HPS_CALL(N) k_invert_blocks<N><<<g, 128, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
+    const int g = (int)((n_nodes + 63) / 64);
+#define HPS_CALL(N) k_invert_blocks<N><<<g, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
     return true;
@@ -483,64 +629,6 @@ namespace hps
     k_to_blocks<<<grid_for(n_nodes * nc, 256 * 4), 256, 0, s>>>(n_nodes, nc, blocks_dev, x);
   }
   // ---- FP32 storage, FP64 accumulation (mixed-precision Krylov basis)
-  template <int KK>
-  __global__ void __launch_bounds__(RB)
-    k_multi_dot_f32(int64_t n, int k, const float *__restrict__ V, const float *__restrict__ w,
-                    double *__restrict__ partial)
-  {
-    __shared__ double sh[RB / 64];
-    double            acc[KK];
-#pragma unroll
-    for (int j = 0; j < KK; ++j)
-      acc[j] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB)
-      {
-        const double wi = w[i];
-#pragma unroll
-        for (int j = 0; j < KK; ++j)
-          if (j < k)
-            acc[j] += (double)V[(int64_t)j * n + i] * wi;
-      }
-#pragma unroll
-    for (int j = 0; j < KK; ++j)
-      if (j < k)
-        {
-          const double s = block_reduce(acc[j], sh);
-          if (threadIdx.x == 0)
-            partial[(int64_t)blockIdx.x * k + j] = s;
-        }
-  }
-
-  template <int KK>
-  __global__ void __launch_bounds__(RB)
-    k_multi_axpy_norm_f32(int64_t n, int k, const float *__restrict__ V,
-                          const double *__restrict__ h, float *__restrict__ w,
-                          double *__restrict__ partial)
-  {
-    __shared__ double sh[RB / 64];
-    double            hk[KK];
-#pragma unroll
-    for (int j = 0; j < KK; ++j)
-      hk[j] = j < k ? h[j] : 0.0;
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB)
-      {
-        double wi = w[i];
-#pragma unroll
-        for (int j = 0; j < KK; ++j)
-          if (j < k)
-            wi -= hk[j] * (double)V[(int64_t)j * n + i];
-        const float wf = (float)wi;
-        w[i]           = wf;
-        s += (double)wf * (double)wf;
-      }
-    s = block_reduce(s, sh);
-    if (threadIdx.x == 0)
-      partial[blockIdx.x] = s;
-  }
-
   __global__ void __launch_bounds__(RB)
     k_norm_f32(int64_t n, const float *__restrict__ x, double *__restrict__ partial)
   {
@@ -561,42 +649,16 @@ namespace hps
       x[i] = (float)(a * (double)x[i]);
   }
 
-  // y (FP64) += sum_j a_j x_j (FP32), j < k
-  __global__ void k_axpy_multi_f32_f64(int64_t n, int k, const double *__restrict__ a,
-                                       const float *__restrict__ X, double *__restrict__ y)
-  {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-      {
-        double s = y[i];
-        for (int j = 0; j < k; ++j)
-          s += a[j] * (double)X[(int64_t)j * n + i];
-        y[i] = s;
-      }
-  }
-
   void blas_multi_dot_f32(hipStream_t s, int64_t n, int k, const float *V, const float *w,
                           double *partial, double *out)
   {
-    if (k <= 8)
-      k_multi_dot_f32<8><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
-    else if (k <= 16)
-      k_multi_dot_f32<16><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
-    else
-      k_multi_dot_f32<MAXK><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, w, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, k, partial, out, 0);
+    multi_dot_any<float>(s, n, k, V, w, partial, out);
   }
 
   void blas_multi_axpy_norm_f32(hipStream_t s, int64_t n, int k, const float *V,
                                 const double *h, float *w, double *partial, double *out)
   {
-    if (k <= 8)
-      k_multi_axpy_norm_f32<8><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
-    else if (k <= 16)
-      k_multi_axpy_norm_f32<16><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
-    else
-      k_multi_axpy_norm_f32<MAXK><<<RED_BLOCKS, RB, 0, s>>>(n, k, V, h, w, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+    multi_axpy_norm_any<float>(s, n, k, V, h, w, partial, out);
   }
 
   void blas_norm2_f32(hipStream_t s, int64_t n, const float *x, double *partial, double *out)
@@ -613,6 +675,8 @@ namespace hps
   void blas_axpy_multi_f32_f64(hipStream_t s, int64_t n, int k, const double *a, const float *X,
                                double *y)
   {
-    k_axpy_multi_f32_f64<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, k, a, X, y);
+#define HPS_CALL(K) k_axpy_multi<K, float><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, X, y)
+    HPS_K_SWITCH(k, HPS_CALL)
+#undef HPS_CALL
   }
 } // namespace hps

@@ -7,7 +7,7 @@
 
 namespace hps
 {
-  constexpr int RED_BLOCKS = 1024; // fixed reduction grid (deterministic sums)
+  constexpr int RED_BLOCKS = 2048; // fixed reduction grid (deterministic sums)
   constexpr int MAXK       = 32;   // max vectors of one multi-dot
 
   void blas_dot(hipStream_t, int64_t n, const double *x, const double *y, double *partial,

@@ -31,9 +31,11 @@ def collect_trajectory(op, u0, n_steps, dt, nl_rel_tol=1e-8, l_rel_tol=1e-2,
     return traj
 
 
-def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
+def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0,
+                  column_weights=(1.0, 1.0, 1.0, 1.0)):
     """rows of (features, targets) from every triple of traj; sets
-    pred.out_scale to the RMS of each target column and normalises by it"""
+    pred.out_scale to the RMS of each target column divided by its weight
+    and normalises by it (a column of weight w counts w^2 in the loss)"""
     op = pred.op
     n = op.n_nodes
     dev = pred.X.device
@@ -52,20 +54,24 @@ def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0):
         Xs.append(X.index_select(0, idx))
         Ts.append(T.index_select(0, idx))
     X, T = torch.cat(Xs), torch.cat(Ts)
-    scale = T.double().pow(2).mean(dim=0).sqrt().clamp_min(1e-12).float()
+    scale = T.double().pow(2).mean(dim=0).sqrt().clamp_min(1e-12)
+    scale = (scale / torch.tensor(column_weights, dtype=torch.float64, device=dev)).float()
     T /= scale
     pred.out_scale.copy_(scale)
     return X, T
 
 
 def fit_predictor(op, traj, dt, iterations=1000, rows_per_step=32768, batch=16384,
-                  lr=1e-3, seed=0, log=None, skip=2):
+                  lr=1e-3, seed=0, log=None, skip=2, column_weights=(1.0, 1.0, 1.0, 1.0)):
     """trains on the triples of traj[skip:] (the first steps of a run are the
-    relaxation of the initial condition and are left out)"""
+    relaxation of the initial condition and are left out).  column_weights
+    weight the outputs (c, mu, eta_top1, eta_top2) in the loss: the c
+    error enters the residual through the mass term w0 = 1/dt and dominates
+    it, so weighting c up lowers the residual of the prediction."""
     traj = traj[skip:]
     mlp = MLP(op, seed=seed)
     pred = LearnedPredictor(op, mlp, dt_scale=1.0 / dt)
-    X, T = build_dataset(pred, traj, dt, rows_per_step, seed)
+    X, T = build_dataset(pred, traj, dt, rows_per_step, seed, column_weights)
     m = X.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed + 1)
     b = min(batch, m)

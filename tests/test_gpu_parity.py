@@ -165,7 +165,8 @@ def test_vmult_f32_error_per_field(dim, cells, G, tens):
 
 
 @pytest.mark.parametrize("dim,cells,G,tens", [
-    (3, (9, 5, 17), 2, False), (3, (5, 9, 20), 2, True), (2, (65, 17), 2, False)])
+    (3, (9, 5, 17), 2, False), (3, (5, 9, 20), 2, True), (2, (65, 17), 2, False),
+    (3, (7, 6, 9), 4, False), (3, (5, 4, 6), 8, False)])
 def test_diagonal_and_node_blocks_match_oracle(dim, cells, G, tens):
     op, p = make_pair(dim, cells, G, tens, w0=1000.0)
     u = fields(p, 6)
@@ -282,3 +283,41 @@ def test_fp32_storage_blas_accumulates_in_fp64():
     np.testing.assert_allclose(host(y), np.array(list(a)) @ V64, rtol=1e-14, atol=1e-14)
     _ffi.check(L.hps_scale_f32(h, n, 0.5, P(dw)))
     np.testing.assert_allclose(host(dw), wr * np.float32(0.5), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("k", list(range(1, 33)))
+def test_multi_vector_kernels_every_k(k):
+    """each basis size k has its own kernel: FP64 and FP32-storage multi-dot,
+    fused update + norm and the multi-axpy against numpy (n odd, so the
+    unrolled steps leave a remainder)"""
+    mesh = CartesianMesh((7, 5, 3), (0, 0, 0), (7.0, 5.0, 3.0))
+    op = SinteringOperatorGeneric(mesh, 3)
+    h, L = op.handle, _ffi.lib()
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    n = 100003
+    rng = np.random.default_rng(k)
+    V = rng.standard_normal((k, n))
+    w = rng.standard_normal(n)
+    r = ctypes.c_double()
+    hh = (ctypes.c_double * k)()
+    dV, dw = dev(V.ravel()), dev(w)
+    _ffi.check(L.hps_multi_dot(h, n, k, P(dV), P(dw), hh))
+    np.testing.assert_allclose(list(hh), V @ w, rtol=1e-12, atol=1e-11)
+    _ffi.check(L.hps_multi_axpy_norm(h, n, k, P(dV), hh, P(dw), ctypes.byref(r)))
+    wr = w - V.T @ np.array(list(hh))
+    np.testing.assert_allclose(host(dw), wr, rtol=1e-12, atol=1e-12)
+    assert abs(r.value - np.linalg.norm(wr)) <= 1e-12 * np.linalg.norm(wr)
+    V32, w32 = V.astype(np.float32), w.astype(np.float32)
+    dV, dw = dev(V32.ravel(), torch.float32), dev(w32, torch.float32)
+    _ffi.check(L.hps_multi_dot_f32(h, n, k, P(dV), P(dw), hh))
+    ref = V32.astype(np.float64) @ w32.astype(np.float64)
+    np.testing.assert_allclose(list(hh), ref, rtol=1e-12, atol=1e-11)
+    _ffi.check(L.hps_multi_axpy_norm_f32(h, n, k, P(dV), hh, P(dw), ctypes.byref(r)))
+    wr = (w32.astype(np.float64) - V32.astype(np.float64).T @ np.array(list(hh))).astype(np.float32)
+    np.testing.assert_allclose(host(dw), wr, rtol=1e-6, atol=1e-6)
+    assert abs(r.value - np.linalg.norm(wr.astype(np.float64))) <= 1e-12 * np.linalg.norm(wr)
+    a = (ctypes.c_double * k)(*rng.standard_normal(k))
+    y = dev(np.ones(n))
+    _ffi.check(L.hps_axpy_multi_f32(h, n, k, a, P(dV), P(y)))
+    np.testing.assert_allclose(host(y), 1.0 + np.array(list(a)) @ V32.astype(np.float64),
+                               rtol=1e-13, atol=1e-13)
