@@ -39,9 +39,9 @@ def parse():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--timesteps", type=int, default=5,
                     help="held-out timesteps timed per method (0 = skip)")
-    ap.add_argument("--train-steps", type=int, default=15,
+    ap.add_argument("--train-steps", type=int, default=30,
                     help="plain-solver steps that train the predictor")
-    ap.add_argument("--train-iterations", type=int, default=1500)
+    ap.add_argument("--train-iterations", type=int, default=6000)
     ap.add_argument("--q2-cells", type=int, default=96,
                     help="cells per direction of the Q2 tensorial workload (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -138,7 +138,8 @@ def timestep_comparison(op, u0, dt, args):
     t0 = time.perf_counter()
     pred = fit_predictor(op, traj[:args.train_steps + 1], dt,
                          iterations=args.train_iterations,
-                         rows_per_step=min(32768, op.n_nodes), seed=0)
+                         rows_per_step=min(131072, op.n_nodes), seed=0,
+                         column_weights=(3.0, 1.0, 1.0, 1.0))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
 

@@ -115,6 +115,7 @@ _SIGNATURES = {
     "hps_vmult_f64": [vp, vp, vp],
     "hps_vmult_f32": [vp, vp, vp],
     "hps_precond_update": [vp, cint],
+    "hps_precond_update_f32": [vp, cint],
     "hps_precond_apply_f64": [vp, vp, vp],
     "hps_precond_apply_f32": [vp, vp, vp],
     "hps_precond_get_diagonal": [vp, vp],

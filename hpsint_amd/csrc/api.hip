@@ -468,7 +468,9 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
   return HPS_OK;
 }
 
-extern "C" int hps_precond_update(hps_operator op, int kind)
+// diagonal / node blocks assembled by probing (FP64 cache and arithmetic,
+// or FP32 for the FP32 inner solve), then inverted in FP64
+static int precond_update(hps_operator op, int kind, bool fp32)
 {
   if (int r = check_op(op))
     return r;
@@ -476,6 +478,7 @@ extern "C" int hps_precond_update(hps_operator op, int kind)
     return fail(HPS_ERR_NOT_INITIALIZED, "preconditioner update before hps_set_linearization");
   const int64_t n = op->n_dofs, nn = op->n_nodes;
   const int     nc = op->nc;
+  fp32 = fp32 && !op->q2; // the Q2 kernels assemble in FP64
   if (kind == HPS_PRECON_POINT_JACOBI)
     {
       if (!op->diag)
@@ -486,11 +489,20 @@ extern "C" int hps_precond_update(hps_operator op, int kind)
         }
       if (op->q2)
         op->q2->probe(q2_params(op), op->cache64, op->diag, -1, nc, 1, 0, op->stream);
+      else if (fp32)
+        op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_diag32, (float *)op->part,
+                         -1, nc, 1, 0, op->stream);
       else
         op->ops->probe(op->kp, op->n_tiles, op->cache64, op->diag, op->part, -1, nc, 1, 0,
                        op->stream);
       HPS_LAUNCHED();
-      if (int r = finish_fixup<double>(op, op->diag, nc, 1, 0))
+      if (fp32)
+        {
+          if (int r = finish_fixup<float>(op, op->inv_diag32, nc, 1, 0))
+            return r;
+          hps::blas_f32_to_f64(op->stream, n, op->diag, op->inv_diag32);
+        }
+      else if (int r = finish_fixup<double>(op, op->diag, nc, 1, 0))
         return r;
       hps::invert_diag(op->stream, n, op->diag, op->inv_diag64, op->inv_diag32);
       HPS_LAUNCHED();
@@ -508,13 +520,23 @@ extern "C" int hps_precond_update(hps_operator op, int kind)
           if (op->q2)
             op->q2->probe(q2_params(op), op->cache64, op->blocks, j, nc * nc, nc, j,
                           op->stream);
+          else if (fp32)
+            op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_blocks32,
+                             (float *)op->part, j, nc * nc, nc, j, op->stream);
           else
             op->ops->probe(op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc,
                            nc, j, op->stream);
           HPS_LAUNCHED();
-          if (int r = finish_fixup<double>(op, op->blocks, nc * nc, nc, j))
+          if (fp32)
+            {
+              if (int r = finish_fixup<float>(op, op->inv_blocks32, nc * nc, nc, j))
+                return r;
+            }
+          else if (int r = finish_fixup<double>(op, op->blocks, nc * nc, nc, j))
             return r;
         }
+      if (fp32)
+        hps::blas_f32_to_f64(op->stream, nn * nc * nc, op->blocks, op->inv_blocks32);
       HPS_HIP(hipMemsetAsync(op->d_singular, 0, sizeof(int), op->stream));
       if (!hps::invert_blocks(op->stream, nn, nc, op->blocks, op->inv_blocks64,
                               op->inv_blocks32, op->d_singular))
@@ -532,6 +554,16 @@ extern "C" int hps_precond_update(hps_operator op, int kind)
     return fail(HPS_ERR_INVALID_ARGUMENT, "unknown preconditioner kind");
   op->precond_kind = kind;
   return HPS_OK;
+}
+
+extern "C" int hps_precond_update(hps_operator op, int kind)
+{
+  return precond_update(op, kind, false);
+}
+
+extern "C" int hps_precond_update_f32(hps_operator op, int kind)
+{
+  return precond_update(op, kind, true);
 }
 
 extern "C" int hps_precond_apply_f64(hps_operator op, double *dst, const double *src)

@@ -118,29 +118,34 @@ namespace hps
   __device__ __forceinline__ void interp_vg(T u, const T *gsc, T &val, T *grad)
   {
     const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
-    const T p  = xshfl<1>(u);
-    const T A  = sa * u + sb * p;
-    const T Dx = u - p;
-    const T pA = xshfl<2>(A);
-    const T pD = xshfl<2>(Dx);
     if constexpr (DIM == 2)
       {
+        const T p  = xshfl<1>(u);
+        const T A  = sa * u + sb * p;
+        const T Dx = u - p;
+        const T pA = xshfl<2>(A);
+        const T pD = xshfl<2>(Dx);
         val     = sa * A + sb * pA;
         grad[1] = (A - pA) * gsc[1];
         grad[0] = (sa * Dx + sb * pD) * gsc[0];
       }
     else
       {
+        const T p    = xshfl<4>(u);
+        const T A    = sa * u + sb * p;
+        const T Dz   = u - p;
+        const T pA   = xshfl<2>(A);
+        const T pDz  = xshfl<2>(Dz);
         const T AA   = sa * A + sb * pA;
         const T Ay   = A - pA;
-        const T DxA  = sa * Dx + sb * pD;
-        const T pAA  = xshfl<4>(AA);
-        const T pAy  = xshfl<4>(Ay);
-        const T pDxA = xshfl<4>(DxA);
+        const T DzA  = sa * Dz + sb * pDz;
+        const T pAA  = xshfl<1>(AA);
+        const T pAy  = xshfl<1>(Ay);
+        const T pDzA = xshfl<1>(DzA);
         val          = sa * AA + sb * pAA;
-        grad[2]      = (AA - pAA) * gsc[2];
+        grad[0]      = (AA - pAA) * gsc[0];
         grad[1]      = (sa * Ay + sb * pAy) * gsc[1];
-        grad[0]      = (sa * DxA + sb * pDxA) * gsc[0];
+        grad[2]      = (sa * DzA + sb * pDzA) * gsc[2];
       }
   }
 
@@ -176,21 +181,21 @@ namespace hps
       }
     else
       {
-        const T pv  = xshfl<4>(v);
-        const T pgz = xshfl<4>(g[2]);
-        const T pgy = xshfl<4>(g[1]);
-        const T pgx = xshfl<4>(g[0]);
-        const T T1  = sa * v + (sb * pv + (g[2] - pgz));
+        const T pv  = xshfl<1>(v);
+        const T pgx = xshfl<1>(g[0]);
+        const T pgy = xshfl<1>(g[1]);
+        const T pgz = xshfl<1>(g[2]);
+        const T T1  = sa * v + (sb * pv + (g[0] - pgx));
         const T Ty  = sa * g[1] + sb * pgy;
-        const T Tx  = sa * g[0] + sb * pgx;
+        const T Tz  = sa * g[2] + sb * pgz;
         const T pT1 = xshfl<2>(T1);
         const T pTy = xshfl<2>(Ty);
-        const T pTx = xshfl<2>(Tx);
+        const T pTz = xshfl<2>(Tz);
         const T U1  = sa * T1 + (sb * pT1 + (Ty - pTy));
-        const T Ux  = sa * Tx + sb * pTx;
-        const T pU1 = xshfl<1>(U1);
-        const T pUx = xshfl<1>(Ux);
-        return sa * U1 + (sb * pU1 + (Ux - pUx));
+        const T Uz  = sa * Tz + sb * pTz;
+        const T pU1 = xshfl<4>(U1);
+        const T pUz = xshfl<4>(Uz);
+        return sa * U1 + (sb * pU1 + (Uz - pUz));
       }
   }
 
@@ -585,10 +590,12 @@ namespace hps
   // operator to unit vectors of the cell's local dofs -- the recipe of
   // MatrixFreeTools::compute_diagonal (used by OperatorBase::
   // compute_inverse_diagonal, S/operator_base.h:364-389).
-  template <int DIM, int TX, int TY, int G, bool TENS>
+  // T = float assembles from the FP32 cache in FP32 arithmetic (the
+  // preconditioner of the FP32 inner solve)
+  template <int DIM, int TX, int TY, int G, bool TENS, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
-    q1_probe(const KParams kp, const double *__restrict__ cache, double *__restrict__ dst,
-             double *__restrict__ part, int column, int ostride, int ocstride, int ooffset)
+    q1_probe(const KParams kp, const T *__restrict__ cache, T *__restrict__ dst,
+             T *__restrict__ part, int column, int ostride, int ocstride, int ooffset)
   {
     using TL            = Tile<DIM, TX, TY>;
     using CL            = CacheLayout<DIM, G, TENS>;
@@ -598,13 +605,12 @@ namespace hps
     constexpr int PN    = TL::PN;
     constexpr int ITEMS = (PN * NC + NT - 1) / NT;
     constexpr int NF    = CL::n_fields;
-    using T             = double;
 
     __shared__ T slots[NV][PN * NC];
     const TL     t(kp);
     const int    tid = t.tid;
-    const T      inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
-    const T      jxw      = kp.jxw;
+    const T      inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+    const T      jxw      = T(kp.jxw);
     const T      sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
 
     T carry[ITEMS];

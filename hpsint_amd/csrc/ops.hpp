@@ -26,6 +26,8 @@ namespace hps
     void (*probe)(const KParams &, int n_tiles, const double *cache, double *dst,
                   double *part, int column, int ostride, int ocstride, int ooffset,
                   hipStream_t);
+    void (*probe32)(const KParams &, int n_tiles, const float *cache, float *dst, float *part,
+                    int column, int ostride, int ocstride, int ooffset, hipStream_t);
   };
 
   template <int DIM, int TX, int TY, int G, bool TENS>
@@ -67,13 +69,19 @@ namespace hps
     static void probe(const KParams &kp, int n_tiles, const double *cache, double *dst,
                       double *part, int column, int os, int ocs, int oo, hipStream_t s)
     {
-      q1_probe<DIM, TX, TY, G, TENS>
+      q1_probe<DIM, TX, TY, G, TENS, double>
+        <<<n_tiles, NT, 0, s>>>(kp, cache, dst, part, column, os, ocs, oo);
+    }
+    static void probe32(const KParams &kp, int n_tiles, const float *cache, float *dst,
+                        float *part, int column, int os, int ocs, int oo, hipStream_t s)
+    {
+      q1_probe<DIM, TX, TY, G, TENS, float>
         <<<n_tiles, NT, 0, s>>>(kp, cache, dst, part, column, os, ocs, oo);
     }
     static OpTable table()
     {
       return OpTable{NT,       TX,       TY,   CacheLayout<DIM, G, TENS>::n_fields,
-                     &vmult64, &vmult32, &residual, &fill, &probe};
+                     &vmult64, &vmult32, &residual, &fill, &probe, &probe32};
     }
   };
 

@@ -465,7 +465,8 @@ extern "C" int hps_newton_solve_ref(hps_newton nt, double *x, const double *cons
       HPS_TRY(hps_set_linearization(op, x)); // setup_jacobian
       if (threshold_exceeded || !nt->precond_ready)
         {
-          HPS_TRY(hps_precond_update(op, p.precond));
+          HPS_TRY(p.inner_fp32 ? hps_precond_update_f32(op, p.precond)
+                               : hps_precond_update(op, p.precond));
           nt->precond_ready = true;
           ++s.precond_updates;
         }

@@ -279,8 +279,13 @@ class Preconditioner:
         self.op = op
         self.kind = self.KINDS[kind]
 
-    def do_update(self):
-        check(lib().hps_precond_update(self.op.handle, self.kind))
+    def do_update(self, fp32=False):
+        """fp32: assemble from the FP32 cache in FP32 arithmetic (the
+        preconditioner of the FP32 inner solve)"""
+        if fp32:
+            check(lib().hps_precond_update_f32(self.op.handle, self.kind))
+        else:
+            check(lib().hps_precond_update(self.op.handle, self.kind))
 
     def vmult(self, dst, src):
         if src.dtype == torch.float32:

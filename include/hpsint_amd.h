@@ -123,6 +123,10 @@ int hps_vmult_f32(hps_operator op, float *dst, const float *src);
  * HPS_PRECON_NODE_BLOCK_JACOBI inverts every node's n_c x n_c block
  * (InverseComponentBlockDiagonalMatrix::do_update, Gauss-Jordan). */
 int hps_precond_update(hps_operator op, int kind);
+/* The same setup for the FP32 inner Krylov solve: the diagonal / blocks are
+ * assembled from the FP32 cache in FP32 arithmetic, then inverted in FP64
+ * (Q2 operators assemble in FP64).  Both applies are valid afterwards. */
+int hps_precond_update_f32(hps_operator op, int kind);
 /* PreconditionerBase::vmult */
 int hps_precond_apply_f64(hps_operator op, double *dst, const double *src);
 int hps_precond_apply_f32(hps_operator op, float *dst, const float *src);

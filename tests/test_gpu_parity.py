@@ -321,3 +321,35 @@ def test_multi_vector_kernels_every_k(k):
     _ffi.check(L.hps_axpy_multi_f32(h, n, k, a, P(dV), P(y)))
     np.testing.assert_allclose(host(y), 1.0 + np.array(list(a)) @ V32.astype(np.float64),
                                rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim,cells,G,tens", [
+    (3, (9, 5, 17), 2, False), (3, (7, 6, 9), 4, False), (3, (5, 9, 20), 2, True),
+    (2, (65, 17), 2, False)])
+def test_fp32_preconditioner_assembly(dim, cells, G, tens):
+    """diagonal and node blocks assembled from the FP32 cache in FP32:
+    relative l2 error of the blocks against the FP64 oracle <= 1e-5, and the
+    FP32 block-Jacobi apply solves with the oracle's blocks to 1e-5"""
+    op, p = make_pair(dim, cells, G, tens, w0=1000.0)
+    u = fields(p, 6)
+    blocks_ref = po.node_blocks(p, po.fill_qp(p, u))
+    op.fill_quadrature_point_values(dev(u))
+    P = Preconditioner(op, "InverseComponentBlockDiagonalMatrix")
+    P.do_update(fp32=True)
+    nc = p.n_comp
+    blocks = torch.empty(op.n_nodes * nc * nc, dtype=torch.float64, device="cuda")
+    _ffi.check(_ffi.lib().hps_precond_get_blocks(op.handle, ctypes.c_void_p(blocks.data_ptr())))
+    b = host(blocks).reshape(blocks_ref.shape)
+    assert rel_l2(b, blocks_ref) <= 1e-5, rel_l2(b, blocks_ref)
+    x = np.random.default_rng(9).standard_normal(u.size)
+    y = op.initialize_dof_vector(torch.float32)
+    P.vmult(y, dev(x, torch.float32))
+    yr = np.linalg.solve(blocks_ref, x.reshape(-1, nc)[..., None])[..., 0].ravel()
+    assert rel_l2(host(y), yr) <= 1e-5
+    D = Preconditioner(op, "InverseDiagonalMatrix")
+    D.do_update(fp32=True)
+    inv = op.initialize_dof_vector()
+    op_inv = np.where(np.abs(np.einsum("nii->ni", blocks_ref).ravel()) > 1e-10,
+                      1.0 / np.einsum("nii->ni", blocks_ref).ravel(), 1.0)
+    _ffi.check(_ffi.lib().hps_precond_get_diagonal(op.handle, ctypes.c_void_p(inv.data_ptr())))
+    assert rel_l2(host(inv), op_inv) <= 1e-5
