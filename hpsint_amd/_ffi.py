@@ -136,6 +136,7 @@ _SIGNATURES = {
     "hps_multi_axpy_norm_f32": [vp, i64, cint, vp, pd, vp, pd],
     "hps_scale_f32": [vp, i64, dbl, vp],
     "hps_axpy_multi_f32": [vp, i64, cint, pd, vp, vp],
+    "hps_axpy_multi": [vp, i64, cint, pd, vp, vp],
     "hps_convert_f32_to_f64": [vp, i64, vp, vp],
     "hps_from_blocks": [vp, vp, ctypes.POINTER(vp)],
     "hps_to_blocks": [vp, ctypes.POINTER(vp), vp],

@@ -995,6 +995,24 @@ extern "C" int hps_scale_f32(hps_operator op, int64_t n, double a, float *x)
   return HPS_OK;
 }
 
+extern "C" int hps_axpy_multi(hps_operator op, int64_t n, int k, const double *a,
+                              const double *X, double *y)
+{
+  if (int r = check_op(op))
+    return r;
+  if (k < 1 || k > hps::MAXK)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "axpy_multi: k must be 1..32");
+  for (int j = 0; j < k; ++j)
+    op->red_host[32 + j] = a[j];
+  HPS_HIP(hipMemcpyAsync(op->red_out + 32, op->red_host + 32, k * sizeof(double),
+                         hipMemcpyHostToDevice, op->stream));
+  hps::blas_axpy_multi_f64(op->stream, n, k, op->red_out + 32, X, y);
+  HPS_LAUNCHED();
+  // the coefficients travel through the pinned buffer: wait before reuse
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  return HPS_OK;
+}
+
 extern "C" int hps_axpy_multi_f32(hps_operator op, int64_t n, int k, const double *a,
                                   const float *X, double *y)
 {

@@ -672,6 +672,14 @@ namespace hps
     k_scale_f32<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x);
   }
 
+  void blas_axpy_multi_f64(hipStream_t s, int64_t n, int k, const double *a, const double *X,
+                           double *y)
+  {
+#define HPS_CALL(K) k_axpy_multi<K, double><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, X, y)
+    HPS_K_SWITCH(k, HPS_CALL)
+#undef HPS_CALL
+  }
+
   void blas_axpy_multi_f32_f64(hipStream_t s, int64_t n, int k, const double *a, const float *X,
                                double *y)
   {

@@ -42,6 +42,8 @@ namespace hps
                                 float *w, double *partial, double *out);
   void blas_norm2_f32(hipStream_t, int64_t n, const float *x, double *partial, double *out);
   void blas_scale_f32(hipStream_t, int64_t n, double a, float *x);
+  void blas_axpy_multi_f64(hipStream_t, int64_t n, int k, const double *a, const double *X,
+                           double *y);
   void blas_axpy_multi_f32_f64(hipStream_t, int64_t n, int k, const double *a, const float *X,
                                double *y);
 } // namespace hps

@@ -749,11 +749,13 @@ namespace hps
   // one thread per (shared node, component); the three classes of shared
   // nodes (on a march-direction tile boundary; on a y boundary only; on an x
   // boundary only) are enumerated back to back, x fastest where possible
-  template <typename T>
+  // (I: index type of the thread decode, int when the thread count fits:
+  // 64-bit divisions are an order of magnitude slower)
+  template <typename T, typename I>
   __global__ void __launch_bounds__(256)
     q1_fixup(const FixParams fp, const T *__restrict__ part, T *__restrict__ dst)
   {
-    int64_t   idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    I         idx = (I)blockIdx.x * blockDim.x + threadIdx.x;
     const int c   = (int)(idx % fp.n_comp);
     idx /= fp.n_comp;
     int g[3];
@@ -775,7 +777,7 @@ namespace hps
       }
     else if ((idx -= fp.n_y) < fp.n_x)
       {
-        const int64_t per = fp.nb[0];
+        const I per = fp.nb[0];
         g[0] = (int)(idx % per + 1) * fp.tsz[0];
         idx /= per;
         g[1] = (int)(idx % fp.nn[1]);

@@ -50,7 +50,10 @@ namespace hps
     if (n == 0)
       return;
     const int blocks = (int)((n + 255) / 256);
-    q1_fixup<T><<<blocks, 256, 0, s>>>(fp, part, dst);
+    if ((int64_t)blocks * 256 < ((int64_t)1 << 31))
+      q1_fixup<T, int><<<blocks, 256, 0, s>>>(fp, part, dst);
+    else
+      q1_fixup<T, int64_t><<<blocks, 256, 0, s>>>(fp, part, dst);
   }
   template void launch_fixup<double>(const FixParams &, const double *, double *, hipStream_t);
   template void launch_fixup<float>(const FixParams &, const float *, float *, hipStream_t);

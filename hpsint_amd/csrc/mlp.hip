@@ -50,8 +50,10 @@ namespace hps
            const float *__restrict__ bias, const uint16_t *__restrict__ H, int kslice,
            int64_t split_stride)
   {
-    __shared__ uint16_t As[GBM][GBK + GPAD];
-    __shared__ uint16_t Bs[GBN][GBK + GPAD];
+    __shared__ __align__(16) uint16_t As[GBM][GBK + GPAD];
+    __shared__ __align__(16) uint16_t Bs[GBN][GBK + GPAD];
+    // 16-byte row loads when every row start is 16-byte aligned
+    const bool vec = ((lda | ldb) & 7) == 0 && (((uintptr_t)A | (uintptr_t)B) & 15) == 0;
     const int tid  = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -70,12 +72,16 @@ namespace hps
         if (!TA)
           {
             const int m = tid >> 2, kc = (tid & 3) * 8;
+            const int gm = m0 + m;
+            if (vec && gm < M && k0 + kc + 8 <= kend)
+              *(uint4 *)&As[m][kc] = *(const uint4 *)&A[(int64_t)gm * lda + k0 + kc];
+            else
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-              {
-                const int gm = m0 + m, gk = k0 + kc + j;
-                As[m][kc + j] = (gm < M && gk < kend) ? A[(int64_t)gm * lda + gk] : 0;
-              }
+              for (int j = 0; j < 8; ++j)
+                {
+                  const int gk = k0 + kc + j;
+                  As[m][kc + j] = (gm < M && gk < kend) ? A[(int64_t)gm * lda + gk] : 0;
+                }
           }
         else
           {
@@ -91,12 +97,16 @@ namespace hps
         if (TB)
           {
             const int n = tid >> 2, kc = (tid & 3) * 8;
+            const int gn = n0 + n;
+            if (vec && gn < N && k0 + kc + 8 <= kend)
+              *(uint4 *)&Bs[n][kc] = *(const uint4 *)&B[(int64_t)gn * ldb + k0 + kc];
+            else
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-              {
-                const int gn = n0 + n, gk = k0 + kc + j;
-                Bs[n][kc + j] = (gn < N && gk < kend) ? B[(int64_t)gn * ldb + gk] : 0;
-              }
+              for (int j = 0; j < 8; ++j)
+                {
+                  const int gk = k0 + kc + j;
+                  Bs[n][kc + j] = (gn < N && gk < kend) ? B[(int64_t)gn * ldb + gk] : 0;
+                }
           }
         else
           {
@@ -112,15 +122,10 @@ namespace hps
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 16)
           {
-            bf16x8 a, b;
-            const uint16_t *pa = &As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8];
-            const uint16_t *pb = &Bs[wn * 32 + (lane & 31)][kk + (lane >> 5) * 8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-              {
-                a[j] = __builtin_bit_cast(__bf16, pa[j]);
-                b[j] = __builtin_bit_cast(__bf16, pb[j]);
-              }
+            const uint4 ra = *(const uint4 *)&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8];
+            const uint4 rb = *(const uint4 *)&Bs[wn * 32 + (lane & 31)][kk + (lane >> 5) * 8];
+            const bf16x8 a = __builtin_bit_cast(bf16x8, ra);
+            const bf16x8 b = __builtin_bit_cast(bf16x8, rb);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
           }
         __syncthreads();
@@ -249,77 +254,131 @@ namespace hps
   // ls the node's local increment scale, the RMS of the 135 increments plus
   // a floor; x[270] = dt * dt_scale, x[271] = log10(ls * inc_scale) (the
   // local scale relative to the global one, 1 / inc_scale).
-  __global__ void k_features(FeatParams fp, const double *__restrict__ u_prev,
-                             const double *__restrict__ u_cur, uint16_t *__restrict__ X,
-                             int *__restrict__ top2, float *__restrict__ lscale)
+  // One wave per 16 consecutive nodes, 4 lanes per node: each lane handles
+  // a quarter of the stencil (the increment scale is summed over the 4
+  // lanes), the lanes build the node's feature row in LDS, then the wave
+  // writes the 16 rows as contiguous 16-byte stores.
+  constexpr int FEAT_LANES = 4;
+  constexpr int FEAT_NODES = 64 / FEAT_LANES;
+
+  __device__ __forceinline__ void stencil_node(const FeatParams &fp, int gx, int gy, int gz,
+                                               int dx, int dy, int dz, int64_t &nb)
   {
-    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= fp.n_nodes)
-      return;
-    const int nc = fp.n_comp, G = nc - 2;
-    const int gx = (int)(node % fp.nn[0]);
-    const int gy = (int)((node / fp.nn[0]) % fp.nn[1]);
-    const int gz = (int)(node / ((int64_t)fp.nn[0] * fp.nn[1]));
-    // top-2 order parameters at the node at t_n
-    int    i1 = 0, i2 = G > 1 ? 1 : 0;
-    double e1 = -1e300, e2 = -1e300;
+    const int xx = min(max(gx + dx, 0), fp.nn[0] - 1);
+    const int yy = min(max(gy + dy, 0), fp.nn[1] - 1);
+    const int zz = min(max(gz + dz, 0), fp.nn[2] - 1);
+    nb           = ((int64_t)zz * fp.nn[1] + yy) * fp.nn[0] + xx;
+  }
+
+  __device__ __forceinline__ void node_fields(const double *__restrict__ a, int G, int i1,
+                                              int i2, double *f)
+  {
+    double sq = 0.0;
     for (int i = 0; i < G; ++i)
+      sq += a[2 + i] * a[2 + i];
+    f[0] = a[0];
+    f[1] = a[1];
+    f[2] = sq;
+    f[3] = a[2 + i1];
+    f[4] = a[2 + i2];
+  }
+
+  __global__ void __launch_bounds__(64)
+    k_features(FeatParams fp, const double *__restrict__ u_prev,
+               const double *__restrict__ u_cur, uint16_t *__restrict__ X,
+               int *__restrict__ top2, float *__restrict__ lscale)
+  {
+    __shared__ __align__(16) uint16_t rows[FEAT_NODES * FEAT_STRIDE];
+    const int     sub  = threadIdx.x % FEAT_LANES; // quarter of the stencil
+    const int     slot = threadIdx.x / FEAT_LANES;
+    const int64_t base = (int64_t)blockIdx.x * FEAT_NODES;
+    const int64_t node = base + slot;
+    const bool    live = node < fp.n_nodes;
+    const int     s0 = (27 * sub) / FEAT_LANES, s1 = (27 * (sub + 1)) / FEAT_LANES;
+    const int     nc = fp.n_comp, G = nc - 2;
+    int           gx = 0, gy = 0, gz = 0, i1 = 0, i2 = 0;
+    double        ss = 0.0;
+    if (live)
       {
-        const double e = u_cur[node * nc + 2 + i];
-        if (e > e1)
+        gx = (int)(node % fp.nn[0]);
+        gy = (int)((node / fp.nn[0]) % fp.nn[1]);
+        gz = (int)(node / ((int64_t)fp.nn[0] * fp.nn[1]));
+        // top-2 order parameters at the node at t_n
+        i2        = G > 1 ? 1 : 0;
+        double e1 = -1e300, e2 = -1e300;
+        for (int i = 0; i < G; ++i)
           {
-            e2 = e1;
-            i2 = i1;
-            e1 = e;
-            i1 = i;
-          }
-        else if (e > e2)
-          {
-            e2 = e;
-            i2 = i;
-          }
-      }
-    if (G == 1)
-      i2 = 0;
-    top2[node * 2]     = i1;
-    top2[node * 2 + 1] = i2;
-    uint16_t *x = X + node * FEAT_STRIDE;
-    double    inc[27][5];
-    double    ss = 0.0;
-    int       s  = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx, ++s)
-          {
-            const int xx = min(max(gx + dx, 0), fp.nn[0] - 1);
-            const int yy = min(max(gy + dy, 0), fp.nn[1] - 1);
-            const int zz = min(max(gz + dz, 0), fp.nn[2] - 1);
-            const int64_t nb = ((int64_t)zz * fp.nn[1] + yy) * fp.nn[0] + xx;
-            const double *a = u_cur + nb * nc, *b = u_prev + nb * nc;
-            double sa = 0.0, sb = 0.0;
-            for (int i = 0; i < G; ++i)
+            const double e = u_cur[node * nc + 2 + i];
+            if (e > e1)
               {
-                sa += a[2 + i] * a[2 + i];
-                sb += b[2 + i] * b[2 + i];
+                e2 = e1;
+                i2 = i1;
+                e1 = e;
+                i1 = i;
               }
-            const double fa[5] = {a[0], a[1], sa, a[2 + i1], a[2 + i2]};
-            const double fb[5] = {b[0], b[1], sb, b[2 + i1], b[2 + i2]};
+            else if (e > e2)
+              {
+                e2 = e;
+                i2 = i;
+              }
+          }
+        if (G == 1)
+          i2 = 0;
+        // pass 1: this lane's part of the sum of squares of the increments
+        for (int s = s0; s < s1; ++s)
+          {
+            int64_t nb;
+            stencil_node(fp, gx, gy, gz, s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1, nb);
+            double fa[5], fb[5];
+            node_fields(u_cur + nb * nc, G, i1, i2, fa);
+            node_fields(u_prev + nb * nc, G, i1, i2, fb);
 #pragma unroll
             for (int f = 0; f < 5; ++f)
               {
-                x[s * 10 + f] = f2bf((float)fa[f]);
-                inc[s][f]     = fa[f] - fb[f];
-                ss += inc[s][f] * inc[s][f];
+                const double d = fa[f] - fb[f];
+                ss += d * d;
               }
           }
-    const double ls = sqrt(ss / 135.0) + fp.inc_floor;
-    for (int q = 0; q < 27; ++q)
+      }
+    // sum over the node's lanes in a fixed order: (q0 + q1) + (q2 + q3)
+    ss += __shfl_xor(ss, 1, 64);
+    ss += __shfl_xor(ss, 2, 64);
+    if (live)
+      {
+        const double ls = sqrt(ss / 135.0) + fp.inc_floor;
+        // pass 2: values and scaled increments of this lane's stencil part
+        uint16_t *x = rows + slot * FEAT_STRIDE;
+        for (int s = s0; s < s1; ++s)
+          {
+            int64_t nb;
+            stencil_node(fp, gx, gy, gz, s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1, nb);
+            double fa[5], fb[5];
+            node_fields(u_cur + nb * nc, G, i1, i2, fa);
+            node_fields(u_prev + nb * nc, G, i1, i2, fb);
 #pragma unroll
-      for (int f = 0; f < 5; ++f)
-        x[q * 10 + 5 + f] = f2bf((float)(inc[q][f] / ls));
-    x[270]       = f2bf((float)(fp.dt * fp.dt_scale));
-    x[271]       = f2bf((float)log10(ls * fp.inc_scale));
-    lscale[node] = (float)ls;
+            for (int f = 0; f < 5; ++f)
+              {
+                x[s * 10 + f]     = f2bf((float)fa[f]);
+                x[s * 10 + 5 + f] = f2bf((float)((fa[f] - fb[f]) / ls));
+              }
+          }
+        if (sub == 0)
+          {
+            x[270]             = f2bf((float)(fp.dt * fp.dt_scale));
+            x[271]             = f2bf((float)log10(ls * fp.inc_scale));
+            lscale[node]       = (float)ls;
+            top2[node * 2]     = i1;
+            top2[node * 2 + 1] = i2;
+          }
+      }
+    __syncthreads();
+    // write the block's rows: contiguous, 16 bytes per lane and store
+    const int64_t n_rows = min((int64_t)FEAT_NODES, fp.n_nodes - base);
+    const int     n_vec  = (int)(n_rows * FEAT_STRIDE / 8);
+    const uint4  *src    = (const uint4 *)rows;
+    uint4        *dst    = (uint4 *)(X + base * FEAT_STRIDE);
+    for (int i = threadIdx.x; i < n_vec; i += 64)
+      dst[i] = src[i];
   }
 
   // u_pred = 2 u_n - u_{n-1} + ls * out_scale[k] * y[k] on (c, mu, eta_i1, eta_i2)
@@ -418,8 +477,8 @@ namespace hps
   void mlp_features(hipStream_t s, const FeatParams &fp, const double *u_prev,
                     const double *u_cur, uint16_t *X, int *top2, float *lscale)
   {
-    k_features<<<(int)((fp.n_nodes + 127) / 128), 128, 0, s>>>(fp, u_prev, u_cur, X, top2,
-                                                             lscale);
+    k_features<<<(int)((fp.n_nodes + FEAT_NODES - 1) / FEAT_NODES), 64, 0, s>>>(
+      fp, u_prev, u_cur, X, top2, lscale);
   }
   void mlp_predict(hipStream_t s, int64_t n_nodes, int nc, const double *u_prev,
                    const double *u_cur, const float *Y, const int *top2,

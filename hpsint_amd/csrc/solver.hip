@@ -86,130 +86,6 @@ namespace
     return hps_precond_apply_f64(op, y, x);
   }
 
-  // Restarted GMRES(m) with left preconditioning and classical Gram-Schmidt
-  // (one re-orthogonalisation pass when the new vector loses more than
-  // half of its norm); convergence as deal.II's ReductionControl on the
-  // preconditioned residual: |r_k| <= max(abs_tol, rel_tol |r_0|).
-  // Solves A x = b with x = 0 on entry.  Returns the iteration count in *its.
-  int gmres(hps_newton nt, const double *b, double *x, int *its)
-  {
-    hps_operator op = nt->op;
-    const int64_t n = nt->n;
-    const int     m = nt->prm.l_restart;
-    double       *V = nt->V;
-    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
-    HPS_TRY(hps_fill(op, n, 0.0, x));
-    // r0 = P^-1 b
-    HPS_TRY(apply_prec(nt, b, V));
-    double beta = 0.0;
-    HPS_TRY(hps_norm2(op, n, V, &beta));
-    const double res0   = beta;
-    const double target = std::max(nt->prm.l_abs_tol, nt->prm.l_rel_tol * res0);
-    int          it     = 0;
-    if (beta <= target)
-      {
-        *its = 0;
-        return HPS_OK;
-      }
-    bool first = true;
-    while (true)
-      {
-        if (!first)
-          {
-            // restart: r = P^-1 (b - A x)
-            HPS_TRY(hps_vmult_f64(op, nt->z64, x));
-            HPS_TRY(hps_sadd(op, n, -1.0, 1.0, b, nt->z64));
-            HPS_TRY(apply_prec(nt, nt->z64, V));
-            HPS_TRY(hps_norm2(op, n, V, &beta));
-            if (beta <= target)
-              break;
-          }
-        first = false;
-        HPS_TRY(hps_scale(op, n, 1.0 / beta, V));
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0]  = beta;
-        int j = 0;
-        bool converged = false;
-        for (; j < m; ++j)
-          {
-            double *vj1 = V + (int64_t)(j + 1) * n;
-            HPS_TRY(apply_prec_op(nt, V + (int64_t)j * n, vj1));
-            HPS_TRY(hps_multi_dot(op, n, j + 1, V, vj1, h.data()));
-            double wnorm = 0.0;
-            HPS_TRY(hps_multi_axpy_norm(op, n, j + 1, V, h.data(), vj1, &wnorm));
-            // norm before the projection, sqrt(|w'|^2 + |h|^2) (V orthonormal)
-            double hh = 0.0;
-            for (int i = 0; i <= j; ++i)
-              hh += h[i] * h[i];
-            const double wnorm_before = std::sqrt(wnorm * wnorm + hh);
-            if (wnorm < 0.5 * wnorm_before)
-              {
-                std::vector<double> h2(j + 1);
-                HPS_TRY(hps_multi_dot(op, n, j + 1, V, vj1, h2.data()));
-                HPS_TRY(hps_multi_axpy_norm(op, n, j + 1, V, h2.data(), vj1, &wnorm));
-                for (int i = 0; i <= j; ++i)
-                  h[i] += h2[i];
-              }
-            h[j + 1] = wnorm;
-            // apply previous rotations to the new column
-            for (int i = 0; i < j; ++i)
-              {
-                const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-                h[i + 1]       = -sn[i] * h[i] + cs[i] * h[i + 1];
-                h[i]           = t;
-              }
-            const double den = std::hypot(h[j], h[j + 1]);
-            cs[j]            = den > 0.0 ? h[j] / den : 1.0;
-            sn[j]            = den > 0.0 ? h[j + 1] / den : 0.0;
-            h[j]             = den;
-            h[j + 1]         = 0.0;
-            g[j + 1]         = -sn[j] * g[j];
-            g[j]             = cs[j] * g[j];
-            for (int i = 0; i <= j; ++i)
-              H[i * m + j] = h[i];
-            ++it;
-            const double res = std::fabs(g[j + 1]);
-            if (res <= target)
-              {
-                converged = true;
-                ++j;
-                break;
-              }
-            if (it >= nt->prm.l_max_iter)
-              {
-                ++j;
-                break;
-              }
-            if (wnorm == 0.0)
-              {
-                converged = true;
-                ++j;
-                break;
-              }
-            HPS_TRY(hps_scale(op, n, 1.0 / wnorm, vj1));
-          }
-        // back substitution H y = g, x += V y
-        for (int i = j - 1; i >= 0; --i)
-          {
-            double s = g[i];
-            for (int k = i + 1; k < j; ++k)
-              s -= H[i * m + k] * y[k];
-            y[i] = s / H[i * m + i];
-          }
-        for (int i = 0; i < j; ++i)
-          HPS_TRY(hps_axpy(op, n, y[i], V + (int64_t)i * n, x));
-        if (converged)
-          break;
-        if (it >= nt->prm.l_max_iter)
-          {
-            *its = it;
-            return fail(HPS_ERR_LINEAR_NO_CONVERGENCE,
-                        "GMRES did not converge in " + std::to_string(it) + " iterations");
-          }
-      }
-    *its = it;
-    return HPS_OK;
-  }
   // Givens-rotation bookkeeping shared by both precisions
   struct Arnoldi
   {
@@ -249,6 +125,100 @@ namespace
         }
     }
   };
+
+  // Restarted GMRES(m) with left preconditioning and classical Gram-Schmidt
+  // (one re-orthogonalisation pass when the new vector loses more than
+  // half of its norm); convergence as deal.II's ReductionControl on the
+  // preconditioned residual: |r_k| <= max(abs_tol, rel_tol |r_0|).
+  // Solves A x = b with x = 0 on entry.  Returns the iteration count in *its.
+  int gmres(hps_newton nt, const double *b, double *x, int *its)
+  {
+    hps_operator  op = nt->op;
+    const int64_t n  = nt->n;
+    const int     m  = nt->prm.l_restart;
+    double       *V  = nt->V;
+    Arnoldi       a(m);
+    HPS_TRY(hps_fill(op, n, 0.0, x));
+    // r0 = P^-1 b
+    HPS_TRY(apply_prec(nt, b, V));
+    double beta = 0.0;
+    HPS_TRY(hps_norm2(op, n, V, &beta));
+    const double target = std::max(nt->prm.l_abs_tol, nt->prm.l_rel_tol * beta);
+    int          it     = 0;
+    if (beta <= target)
+      {
+        *its = 0;
+        return HPS_OK;
+      }
+    bool first = true;
+    while (true)
+      {
+        if (!first)
+          {
+            // restart: r = P^-1 (b - A x)
+            HPS_TRY(hps_vmult_f64(op, nt->z64, x));
+            HPS_TRY(hps_sadd(op, n, -1.0, 1.0, b, nt->z64));
+            HPS_TRY(apply_prec(nt, nt->z64, V));
+            HPS_TRY(hps_norm2(op, n, V, &beta));
+            if (beta <= target)
+              break;
+          }
+        first = false;
+        HPS_TRY(hps_scale(op, n, 1.0 / beta, V));
+        std::fill(a.g.begin(), a.g.end(), 0.0);
+        a.g[0]         = beta;
+        int  j         = 0;
+        bool converged = false;
+        for (; j < m; ++j)
+          {
+            double *vj1 = V + (int64_t)(j + 1) * n;
+            HPS_TRY(apply_prec_op(nt, V + (int64_t)j * n, vj1));
+            HPS_TRY(hps_multi_dot(op, n, j + 1, V, vj1, a.h.data()));
+            double wnorm = 0.0;
+            HPS_TRY(hps_multi_axpy_norm(op, n, j + 1, V, a.h.data(), vj1, &wnorm));
+            // norm before the projection, sqrt(|w'|^2 + |h|^2) (V orthonormal)
+            double hh = 0.0;
+            for (int i = 0; i <= j; ++i)
+              hh += a.h[i] * a.h[i];
+            if (wnorm < 0.5 * std::sqrt(wnorm * wnorm + hh))
+              {
+                std::vector<double> h2(j + 1);
+                HPS_TRY(hps_multi_dot(op, n, j + 1, V, vj1, h2.data()));
+                HPS_TRY(hps_multi_axpy_norm(op, n, j + 1, V, h2.data(), vj1, &wnorm));
+                for (int i = 0; i <= j; ++i)
+                  a.h[i] += h2[i];
+              }
+            a.h[j + 1]       = wnorm;
+            const double res = a.add_column(j);
+            ++it;
+            if (res <= target || wnorm == 0.0)
+              {
+                converged = true;
+                ++j;
+                break;
+              }
+            if (it >= nt->prm.l_max_iter)
+              {
+                ++j;
+                break;
+              }
+            HPS_TRY(hps_scale(op, n, 1.0 / wnorm, vj1));
+          }
+        // back substitution H y = g, x += V y
+        a.solve(j);
+        HPS_TRY(hps_axpy_multi(op, n, j, a.y.data(), V, x));
+        if (converged)
+          break;
+        if (it >= nt->prm.l_max_iter)
+          {
+            *its = it;
+            return fail(HPS_ERR_LINEAR_NO_CONVERGENCE,
+                        "GMRES did not converge in " + std::to_string(it) + " iterations");
+          }
+      }
+    *its = it;
+    return HPS_OK;
+  }
 
   // GMRES with the Krylov basis stored in FP32: FP32 tangent and
   // preconditioner, Gram-Schmidt coefficients and norms accumulated in FP64,

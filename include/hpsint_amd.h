@@ -159,6 +159,10 @@ int hps_multi_dot(hps_operator op, int64_t n, int k, const double *V,
 /* w -= sum_j h[j] V_j, then returns ||w|| (fused CGS update + norm) */
 int hps_multi_axpy_norm(hps_operator op, int64_t n, int k, const double *V,
                         const double *h, double *w, double *norm);
+/* y += sum_{j<k} a[j] V_j (the GMRES solution update x += V y; a on the
+ * host, V as in hps_multi_dot) */
+int hps_axpy_multi(hps_operator op, int64_t n, int k, const double *a, const double *V,
+                   double *y);
 /* conversions FP64 <-> FP32 */
 int hps_convert_f64_to_f32(hps_operator op, int64_t n, float *dst,
                            const double *src);

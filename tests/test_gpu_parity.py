@@ -318,6 +318,9 @@ def test_multi_vector_kernels_every_k(k):
     assert abs(r.value - np.linalg.norm(wr.astype(np.float64))) <= 1e-12 * np.linalg.norm(wr)
     a = (ctypes.c_double * k)(*rng.standard_normal(k))
     y = dev(np.ones(n))
+    _ffi.check(L.hps_axpy_multi(h, n, k, a, P(dev(V.ravel())), P(y)))
+    np.testing.assert_allclose(host(y), 1.0 + np.array(list(a)) @ V, rtol=1e-13, atol=1e-13)
+    y = dev(np.ones(n))
     _ffi.check(L.hps_axpy_multi_f32(h, n, k, a, P(dV), P(y)))
     np.testing.assert_allclose(host(y), 1.0 + np.array(list(a)) @ V32.astype(np.float64),
                                rtol=1e-13, atol=1e-13)
