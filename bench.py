@@ -27,6 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP64 vmult main kernel on the 128^3, 4 order
+# parameter workload: FETCH_SIZE and WRITE_SIZE (rocprofv3 --pmc), each
+# scaled by the same counter's reading for a copy of known size; counts and
+# factors in profiles/vmult_q1_128cubed_hbm_traffic.md
+VMULT_TRAFFIC_128 = 1.3841e9
 
 
 def parse():
@@ -319,7 +324,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": VMULT_TRAFFIC_128 if (n == 128 and G == 4) else None,
         },
         "cpu_baseline": cpu,
         "kernel_ms": {"vmult_main": t_main * 1e3, "vmult_fixup": t_fix * 1e3,
