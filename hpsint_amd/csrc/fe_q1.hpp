@@ -51,7 +51,21 @@ namespace hps
     double jxw;       // prod(h) * (1/2)^DIM
     double w_hist[3]; // BDF weights[1..order]
     PhysParams pp;
+    // uniform scales of the folded scalar tangent (FoldScales, set by
+    // set_fold_scales): JxW/h^2, times -kappa_c, times L kappa_p
+    double fold_kj[3], fold_kmu[3], fold_keta[3];
   };
+
+  // fills the uniform folded-tangent scales of kp from jxw, inv_h and pp
+  inline void set_fold_scales(KParams &kp)
+  {
+    for (int d = 0; d < 3; ++d)
+      {
+        kp.fold_kj[d]   = kp.jxw * kp.inv_h[d] * kp.inv_h[d];
+        kp.fold_kmu[d]  = -kp.pp.kappa_c * kp.fold_kj[d];
+        kp.fold_keta[d] = kp.pp.L * kp.pp.kappa_p * kp.fold_kj[d];
+      }
+  }
 
   enum Op
   {
@@ -147,6 +161,14 @@ namespace hps
         grad[1]      = (sa * Ay + sb * pAy) * gsc[1];
         grad[2]      = (sa * DzA + sb * pDzA) * gsc[2];
       }
+  }
+
+  // interp_vg without the per-lane signed 1/h of the gradients
+  template <int DIM, typename T>
+  __device__ __forceinline__ void interp_vg_pre(T u, T &val, T *grad)
+  {
+    const T one[3] = {T(1), T(1), T(1)};
+    interp_vg<DIM>(u, one, val, grad);
   }
 
   template <int DIM, typename T>
@@ -318,6 +340,18 @@ namespace hps
     signed_scales<DIM>(t.corner, inv_h, T(1), gsc);
     signed_scales<DIM>(t.corner, inv_h, T(kp.jxw), gsj);
     const T jxw = T(kp.jxw);
+    // scalar-mobility tangent: constant gradient rows folded into uniform
+    // scales (FoldScales)
+    constexpr bool FOLD = !TENS && OP == OP_VMULT;
+    FoldScales<T>  fs;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        fs.kj[d]   = T(kp.fold_kj[d]);
+        fs.kmu[d]  = T(kp.fold_kmu[d]);
+        fs.keta[d] = T(kp.fold_keta[d]);
+        fs.gsj[d]  = gsj[d];
+      }
 
     T regs[NPL][ITEMS];
     auto load_plane = [&](int k) {
@@ -389,10 +423,23 @@ namespace hps
         T        val[NC], grad[NC][DIM];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-          interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
+          if constexpr (FOLD)
+            interp_vg_pre<DIM>(pl[c], val[c], grad[c]);
+          else
+            interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
 
         T vr[NC], gr[NC][DIM];
-        if constexpr (OP == OP_VMULT)
+        if constexpr (FOLD)
+          {
+            T lin[NF];
+            const T *cl = cache_tile + k * (NF * NT);
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+              lin[f] = cl[f * NT + tid];
+            tangent_qp_rows<DIM, G, false, true, true, T, true>(kp.pp, lin, val, grad, vr, gr,
+                                                                &fs);
+          }
+        else if constexpr (OP == OP_VMULT)
           {
             T lin[NF];
             const T *cl = cache_tile + k * (NF * NT);
@@ -421,7 +468,7 @@ namespace hps
             T gs[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-              gs[d] = gr[c][d] * gsj[d];
+              gs[d] = FOLD ? gr[c][d] : gr[c][d] * gsj[d];
             sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
           }
         __syncthreads();

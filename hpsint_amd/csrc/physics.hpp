@@ -101,14 +101,29 @@ namespace hps
   // ------------------------------------------------------------------
   // VALS / GRADC select the value rows and the gradient row of c; the other
   // gradient rows are always written
-  template <int DIM, int G, bool TENS, bool VALS, bool GRADC, typename T>
-  __device__ __forceinline__ void tangent_qp_rows(const PhysParams &pp,
-                                                  const T          *lin,
-                                                  const T          *v,
-                                                  const T (*g)[DIM],
-                                                  T                *vr,
-                                                  T (*gr)[DIM])
+  // Scales of the folded scalar tangent (FOLD = true): g holds gradients
+  // without their per-lane signed 1/h, and the gradient rows come out as
+  // the integration inputs (times the signed JxW/h).  The two signs cancel,
+  // so a constant gradient row needs one product with a uniform scale:
+  //   kj[d]  = JxW / h_d^2          (grad mu in the c row)
+  //   kmu[d] = -kappa_c JxW / h_d^2 (mu row), keta[d] = L kappa_p JxW / h_d^2
+  //   gsj[d] = signed JxW / h_d     (the cached grad mu of the c row)
+  template <typename T>
+  struct FoldScales
   {
+    T kj[3], kmu[3], keta[3], gsj[3];
+  };
+
+  template <int DIM, int G, bool TENS, bool VALS, bool GRADC, typename T, bool FOLD = false>
+  __device__ __forceinline__ void tangent_qp_rows(const PhysParams     &pp,
+                                                  const T              *lin,
+                                                  const T              *v,
+                                                  const T (*g)[DIM],
+                                                  T                    *vr,
+                                                  T (*gr)[DIM],
+                                                  const FoldScales<T>  *fs = nullptr)
+  {
+    static_assert(!(FOLD && TENS), "the folded form exists for scalar mobility only");
     using CL        = CacheLayout<DIM, G, TENS>;
     const T  c      = lin[CL::c];
     const T *etas   = lin + CL::eta;
@@ -157,9 +172,14 @@ namespace hps
         for (int i = 0; i < G; ++i)
           factor += (eta_sum - etas[i]) * v[2 + i];
         const T s = dMdc * v[0] + (T(2.0) * Mgb) * factor;
+        if constexpr (FOLD)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          gr[0][d] = M * g[1][d] + s * lmu[d];
+          for (int d = 0; d < DIM; ++d)
+            gr[0][d] = (M * fs->kj[d]) * g[1][d] + s * (lmu[d] * fs->gsj[d]);
+        else
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            gr[0][d] = M * g[1][d] + s * lmu[d];
       }
     else
       {
@@ -250,16 +270,30 @@ namespace hps
       }
 
     // 2) mu row
-    const T kc = -T(pp.kappa_c);
+    if constexpr (FOLD)
+      {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      gr[1][d] = kc * g[0][d];
-    const T Lkp = L * T(pp.kappa_p);
+        for (int d = 0; d < DIM; ++d)
+          gr[1][d] = fs->kmu[d] * g[0][d];
 #pragma unroll
-    for (int i = 0; i < G; ++i)
+        for (int i = 0; i < G; ++i)
 #pragma unroll
-      for (int d = 0; d < DIM; ++d)
-        gr[i + 2][d] = Lkp * g[i + 2][d];
+          for (int d = 0; d < DIM; ++d)
+            gr[i + 2][d] = fs->keta[d] * g[i + 2][d];
+      }
+    else
+      {
+        const T kc = -T(pp.kappa_c);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+          gr[1][d] = kc * g[0][d];
+        const T Lkp = L * T(pp.kappa_p);
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            gr[i + 2][d] = Lkp * g[i + 2][d];
+      }
     if constexpr (!VALS)
       return;
     const T d2f_dc2 = T(2.0) * A * (c * c) + T(4.0) * A * c * (T(2.0) * c - T(2.0)) +
