@@ -429,15 +429,30 @@ namespace hps
             interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
 
         T vr[NC], gr[NC][DIM];
+        T *sl = &slots[t.corner][t.pn_lane * NC];
         if constexpr (FOLD)
           {
+            // value rows first, then each component's gradient row is
+            // formed and integrated at once (shorter live ranges)
             T lin[NF];
             const T *cl = cache_tile + k * (NF * NT);
 #pragma unroll
             for (int f = 0; f < NF; ++f)
               lin[f] = cl[f * NT + tid];
-            tangent_qp_rows<DIM, G, false, true, true, T, true>(kp.pp, lin, val, grad, vr, gr,
-                                                                &fs);
+            tangent_qp_rows<DIM, G, false, true, false>(kp.pp, lin, val, grad, vr, gr);
+            T M, s;
+            scalar_mobility_terms<G>(kp.pp, lin[CL::c], lin + CL::eta, val, M, s);
+            const T *lmu = lin + CL::grad_mu;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              {
+                T gs[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  gs[d] = c == 0 ? (M * fs.kj[d]) * grad[1][d] + s * (lmu[d] * fs.gsj[d])
+                                 : (c == 1 ? fs.kmu[d] * grad[0][d] : fs.keta[d] * grad[c][d]);
+                sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
+              }
           }
         else if constexpr (OP == OP_VMULT)
           {
@@ -461,16 +476,16 @@ namespace hps
             residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, val, grad, hist, vr, gr);
           }
 
-        T *sl = &slots[t.corner][t.pn_lane * NC];
+        if constexpr (!FOLD)
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-          {
-            T gs[DIM];
+          for (int c = 0; c < NC; ++c)
+            {
+              T gs[DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              gs[d] = FOLD ? gr[c][d] : gr[c][d] * gsj[d];
-            sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
-          }
+              for (int d = 0; d < DIM; ++d)
+                gs[d] = gr[c][d] * gsj[d];
+              sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
+            }
         __syncthreads();
 
         if (pre)

@@ -114,16 +114,49 @@ namespace hps
     T kj[3], kmu[3], keta[3], gsj[3];
   };
 
-  template <int DIM, int G, bool TENS, bool VALS, bool GRADC, typename T, bool FOLD = false>
-  __device__ __forceinline__ void tangent_qp_rows(const PhysParams     &pp,
-                                                  const T              *lin,
-                                                  const T              *v,
-                                                  const T (*g)[DIM],
-                                                  T                    *vr,
-                                                  T (*gr)[DIM],
-                                                  const FoldScales<T>  *fs = nullptr)
+  // c-row coefficients of the scalar-mobility tangent
+  // (MobilityScalar::apply_M_derivative, S/mobility.h:477-534):
+  //   grad row of c = M grad(dmu) + s grad(mu_lin),
+  //   s = dM/dc dc + 2 Mgb sum_i (sum_j eta_j - eta_i) deta_i
+  template <int G, typename T>
+  __device__ __forceinline__ void scalar_mobility_terms(const PhysParams &pp, T c,
+                                                        const T *etas, const T *v, T &M,
+                                                        T &s)
   {
-    static_assert(!(FOLD && TENS), "the folded form exists for scalar mobility only");
+    const T Mvol  = T(pp.Mvol);
+    const T Mvap  = T(pp.Mvap);
+    const T Msurf = T(pp.Msurf);
+    const T Mgb   = T(pp.Mgb);
+    T       eta_sum = T(0.0), eta_ij_sum = T(0.0);
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+      {
+        eta_sum += etas[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j)
+          eta_ij_sum += etas[i] * etas[j];
+      }
+    const T phi = phi_clamped(c);
+    M           = Mvol * phi + Mvap * (T(1.0) - phi) +
+        Msurf * T(4.0) * c * c * (T(1.0) - c) * (T(1.0) - c) + (T(2.0) * Mgb) * eta_ij_sum;
+    const T dphidc = T(30.0) * c * c * (T(1.0) - c) * (T(1.0) - c);
+    const T dMdc   = Mvol * dphidc - Mvap * dphidc +
+                   Msurf * T(8.0) * c * (T(1.0) - T(3.0) * c + T(2.0) * c * c);
+    T factor = T(0.0);
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+      factor += (eta_sum - etas[i]) * v[2 + i];
+    s = dMdc * v[0] + (T(2.0) * Mgb) * factor;
+  }
+
+  template <int DIM, int G, bool TENS, bool VALS, bool GRADC, typename T>
+  __device__ __forceinline__ void tangent_qp_rows(const PhysParams &pp,
+                                                  const T          *lin,
+                                                  const T          *v,
+                                                  const T (*g)[DIM],
+                                                  T                *vr,
+                                                  T (*gr)[DIM])
+  {
     using CL        = CacheLayout<DIM, G, TENS>;
     const T  c      = lin[CL::c];
     const T *etas   = lin + CL::eta;
@@ -150,36 +183,11 @@ namespace hps
       }
     else if constexpr (!TENS)
       {
-        // MobilityScalar::apply_M_derivative, S/mobility.h:477-534
-        T eta_sum = T(0.0), eta_ij_sum = T(0.0);
+        T M, s;
+        scalar_mobility_terms<G>(pp, c, etas, v, M, s);
 #pragma unroll
-        for (int i = 0; i < G; ++i)
-          {
-            eta_sum += etas[i];
-#pragma unroll
-            for (int j = 0; j < i; ++j)
-              eta_ij_sum += etas[i] * etas[j];
-          }
-        const T phi = phi_clamped(c);
-        const T M   = Mvol * phi + Mvap * (T(1.0) - phi) +
-                    Msurf * T(4.0) * c * c * (T(1.0) - c) * (T(1.0) - c) +
-                    (T(2.0) * Mgb) * eta_ij_sum;
-        const T dphidc = T(30.0) * c * c * (T(1.0) - c) * (T(1.0) - c);
-        const T dMdc   = Mvol * dphidc - Mvap * dphidc +
-                       Msurf * T(8.0) * c * (T(1.0) - T(3.0) * c + T(2.0) * c * c);
-        T factor = T(0.0);
-#pragma unroll
-        for (int i = 0; i < G; ++i)
-          factor += (eta_sum - etas[i]) * v[2 + i];
-        const T s = dMdc * v[0] + (T(2.0) * Mgb) * factor;
-        if constexpr (FOLD)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            gr[0][d] = (M * fs->kj[d]) * g[1][d] + s * (lmu[d] * fs->gsj[d]);
-        else
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            gr[0][d] = M * g[1][d] + s * lmu[d];
+        for (int d = 0; d < DIM; ++d)
+          gr[0][d] = M * g[1][d] + s * lmu[d];
       }
     else
       {
@@ -270,30 +278,16 @@ namespace hps
       }
 
     // 2) mu row
-    if constexpr (FOLD)
-      {
+    const T kc = -T(pp.kappa_c);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          gr[1][d] = fs->kmu[d] * g[0][d];
+    for (int d = 0; d < DIM; ++d)
+      gr[1][d] = kc * g[0][d];
+    const T Lkp = L * T(pp.kappa_p);
 #pragma unroll
-        for (int i = 0; i < G; ++i)
+    for (int i = 0; i < G; ++i)
 #pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            gr[i + 2][d] = fs->keta[d] * g[i + 2][d];
-      }
-    else
-      {
-        const T kc = -T(pp.kappa_c);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          gr[1][d] = kc * g[0][d];
-        const T Lkp = L * T(pp.kappa_p);
-#pragma unroll
-        for (int i = 0; i < G; ++i)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            gr[i + 2][d] = Lkp * g[i + 2][d];
-      }
+      for (int d = 0; d < DIM; ++d)
+        gr[i + 2][d] = Lkp * g[i + 2][d];
     if constexpr (!VALS)
       return;
     const T d2f_dc2 = T(2.0) * A * (c * c) + T(4.0) * A * c * (T(2.0) * c - T(2.0)) +
