@@ -38,7 +38,33 @@ namespace hps
     return __uint_as_float(((uint32_t)b) << 16);
   }
 
-  constexpr int GBM = 64, GBN = 64, GBK = 32, GPAD = 8; // tile and LDS padding
+  // tanh for the hidden layers (the result is rounded to bf16, relative
+  // spacing 2^-8): the odd Taylor polynomial through x^9 for |x| < 0.5
+  // (relative error below 1e-5, largest at |x| = 0.5), and
+  // 1 - 2 / (exp(2|x|) + 1) with the hardware exponential elsewhere
+  // (relative error below 1e-6)
+  __device__ __forceinline__ float tanh_fast(float x)
+  {
+    const float a = fabsf(x);
+    float       r;
+    if (a < 0.5f)
+      {
+        const float x2 = x * x;
+        // x - x^3/3 + 2x^5/15 - 17x^7/315 + 62x^9/2835
+        r = x * (1.0f + x2 * (-0.333333333f +
+                               x2 * (0.133333333f + x2 * (-0.0539682540f + x2 * 0.0218694885f))));
+      }
+    else
+      {
+        const float e = __expf(2.0f * a);
+        r             = copysignf(1.0f - 2.0f / (e + 1.0f), x);
+      }
+    return r;
+  }
+
+  // tile: 64 rows x 128 columns (a 128-wide layer reads its input once),
+  // 4 waves as 2 x 2, each 32 rows x 64 columns = two 32x32 MFMA tiles
+  constexpr int GBM = 64, GBN = 128, GBK = 32, GPAD = 8;
 
   // C[m][n] = sum_k A(m,k) B(k,n);  A(m,k) = TA ? A[k*lda+m] : A[m*lda+k],
   // B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n].  blockIdx.z selects a K-slice of
@@ -61,91 +87,169 @@ namespace hps
     const int kbeg = blockIdx.z * kslice;
     const int kend = min(K, kbeg + kslice);
 
-    f32x16 acc;
+    f32x16 acc[2];
 #pragma unroll
-    for (int r = 0; r < 16; ++r)
-      acc[r] = 0.f;
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        acc[t][r] = 0.f;
 
-    for (int k0 = kbeg; k0 < kend; k0 += GBK)
+    // 8 consecutive bf16 from p, the first `valid` of them (the rest zero)
+    auto load8 = [](const uint16_t *p, int valid) -> uint4 {
+      if (valid >= 8)
+        return *(const uint4 *)p;
+      uint16_t e[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        e[j] = j < valid ? p[j] : 0;
+      uint4 r;
+      r.x = e[0] | ((uint32_t)e[1] << 16);
+      r.y = e[2] | ((uint32_t)e[3] << 16);
+      r.z = e[4] | ((uint32_t)e[5] << 16);
+      r.w = e[6] | ((uint32_t)e[7] << 16);
+      return r;
+    };
+    if (!TA && TB && vec)
       {
-        // ---- stage A tile (GBM x GBK) as As[m][k]
-        if (!TA)
-          {
-            const int m = tid >> 2, kc = (tid & 3) * 8;
-            const int gm = m0 + m;
-            if (vec && gm < M && k0 + kc + 8 <= kend)
-              *(uint4 *)&As[m][kc] = *(const uint4 *)&A[(int64_t)gm * lda + k0 + kc];
-            else
+        // row-major A and row-major B^T (the inference layout): the next
+        // K-tile is fetched into registers while the current one is on MFMA
+        const int am = tid >> 2, kc = (tid & 3) * 8;
+        uint4     pa, pb[2];
+        auto      fetch = [&](int k0) {
+          const int valid = min(8, max(0, kend - (k0 + kc)));
+          const int gm    = m0 + am;
+          pa = gm < M ? load8(&A[(int64_t)gm * lda + k0 + kc], valid) : make_uint4(0, 0, 0, 0);
 #pragma unroll
-              for (int j = 0; j < 8; ++j)
-                {
-                  const int gk = k0 + kc + j;
-                  As[m][kc + j] = (gm < M && gk < kend) ? A[(int64_t)gm * lda + gk] : 0;
-                }
-          }
-        else
+          for (int j = 0; j < 2; ++j)
+            {
+              const int gn = n0 + (tid >> 2) + j * 64;
+              pb[j] = gn < N ? load8(&B[(int64_t)gn * ldb + k0 + kc], valid)
+                             : make_uint4(0, 0, 0, 0);
+            }
+        };
+        fetch(kbeg);
+        for (int k0 = kbeg; k0 < kend; k0 += GBK)
           {
-            const int k = tid >> 3, mc = (tid & 7) * 8;
+            *(uint4 *)&As[am][kc] = pa;
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
+            for (int j = 0; j < 2; ++j)
+              *(uint4 *)&Bs[(tid >> 2) + j * 64][kc] = pb[j];
+            __syncthreads();
+            if (k0 + GBK < kend)
+              fetch(k0 + GBK);
+#pragma unroll
+            for (int kk = 0; kk < GBK; kk += 16)
               {
-                const int gm = m0 + mc + j, gk = k0 + k;
-                As[mc + j][k] = (gm < M && gk < kend) ? A[(int64_t)gk * lda + gm] : 0;
+                const uint4 ra =
+                  *(const uint4 *)&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8];
+                const bf16x8 a = __builtin_bit_cast(bf16x8, ra);
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                  {
+                    const uint4 rb =
+                      *(const uint4 *)&Bs[wn * 64 + t * 32 + (lane & 31)][kk + (lane >> 5) * 8];
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, rb);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+                  }
               }
+            __syncthreads();
           }
-        // ---- stage B tile (GBK x GBN) as Bs[n][k]
-        if (TB)
-          {
-            const int n = tid >> 2, kc = (tid & 3) * 8;
-            const int gn = n0 + n;
-            if (vec && gn < N && k0 + kc + 8 <= kend)
-              *(uint4 *)&Bs[n][kc] = *(const uint4 *)&B[(int64_t)gn * ldb + k0 + kc];
-            else
-#pragma unroll
-              for (int j = 0; j < 8; ++j)
-                {
-                  const int gk = k0 + kc + j;
-                  Bs[n][kc + j] = (gn < N && gk < kend) ? B[(int64_t)gn * ldb + gk] : 0;
-                }
-          }
-        else
-          {
-            const int k = tid >> 3, nc = (tid & 7) * 8;
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-              {
-                const int gn = n0 + nc + j, gk = k0 + k;
-                Bs[nc + j][k] = (gn < N && gk < kend) ? B[(int64_t)gk * ldb + gn] : 0;
-              }
-          }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GBK; kk += 16)
-          {
-            const uint4 ra = *(const uint4 *)&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8];
-            const uint4 rb = *(const uint4 *)&Bs[wn * 32 + (lane & 31)][kk + (lane >> 5) * 8];
-            const bf16x8 a = __builtin_bit_cast(bf16x8, ra);
-            const bf16x8 b = __builtin_bit_cast(bf16x8, rb);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-          }
-        __syncthreads();
       }
+    else
+  for (int k0 = kbeg; k0 < kend; k0 += GBK)
+        {
+          // ---- stage A tile (GBM x GBK) as As[m][k]
+          if (!TA)
+            {
+              const int m = tid >> 2, kc = (tid & 3) * 8;
+              const int gm = m0 + m;
+              if (vec && gm < M && k0 + kc + 8 <= kend)
+                *(uint4 *)&As[m][kc] = *(const uint4 *)&A[(int64_t)gm * lda + k0 + kc];
+              else
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                  {
+                    const int gk = k0 + kc + j;
+                    As[m][kc + j] = (gm < M && gk < kend) ? A[(int64_t)gm * lda + gk] : 0;
+                  }
+            }
+          else
+            {
+              const int k = tid >> 3, mc = (tid & 7) * 8;
+#pragma unroll
+              for (int j = 0; j < 8; ++j)
+                {
+                  const int gm = m0 + mc + j, gk = k0 + k;
+                  As[mc + j][k] = (gm < M && gk < kend) ? A[(int64_t)gk * lda + gm] : 0;
+                }
+            }
+          // ---- stage B tile (GBK x GBN) as Bs[n][k], 8 consecutive k or n per item
+          if (TB)
+            {
+              for (int ch = tid; ch < GBN * GBK / 8; ch += 256)
+                {
+                  const int n = ch >> 2, kc = (ch & 3) * 8;
+                  const int gn = n0 + n;
+                  if (vec && gn < N && k0 + kc + 8 <= kend)
+                    *(uint4 *)&Bs[n][kc] = *(const uint4 *)&B[(int64_t)gn * ldb + k0 + kc];
+                  else
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                      {
+                        const int gk = k0 + kc + j;
+                        Bs[n][kc + j] = (gn < N && gk < kend) ? B[(int64_t)gn * ldb + gk] : 0;
+                      }
+                }
+            }
+          else
+            {
+              for (int ch = tid; ch < GBN * GBK / 8; ch += 256)
+                {
+                  const int k = ch / (GBN / 8), nc = (ch % (GBN / 8)) * 8;
+#pragma unroll
+                  for (int j = 0; j < 8; ++j)
+                    {
+                      const int gn = n0 + nc + j, gk = k0 + k;
+                      Bs[nc + j][k] = (gn < N && gk < kend) ? B[(int64_t)gk * ldb + gn] : 0;
+                    }
+                }
+            }
+          __syncthreads();
+#pragma unroll
+          for (int kk = 0; kk < GBK; kk += 16)
+            {
+              const uint4  ra = *(const uint4 *)&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8];
+              const bf16x8 a  = __builtin_bit_cast(bf16x8, ra);
+#pragma unroll
+              for (int t = 0; t < 2; ++t)
+                {
+                  const uint4 rb =
+                    *(const uint4 *)&Bs[wn * 64 + t * 32 + (lane & 31)][kk + (lane >> 5) * 8];
+                  const bf16x8 b = __builtin_bit_cast(bf16x8, rb);
+                  acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+                }
+            }
+          __syncthreads();
+        }
 
+  
     // ---- epilogue: acc[r] holds C[row][col] with
     //      row = 8*(r/4) + 4*(lane/32) + r%4, col = lane%32 (within the 32x32 tile)
-    const int col = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int r = 0; r < 16; ++r)
       {
+        const int col = n0 + wn * 64 + t * 32 + (lane & 31);
         const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         if (row >= M || col >= N)
           continue;
-        float v = acc[r];
+        float v = acc[t][r];
         const int64_t o = (int64_t)row * ldc + col;
         if constexpr (EPI == EPI_F32)
           ((float *)Cout)[blockIdx.z * split_stride + o] = v;
         else if constexpr (EPI == EPI_BIAS_TANH)
-          ((uint16_t *)Cout)[o] = f2bf(tanhf(v + bias[col]));
+          ((uint16_t *)Cout)[o] = f2bf(tanh_fast(v + bias[col]));
         else if constexpr (EPI == EPI_BIAS_F32)
           ((float *)Cout)[o] = v + bias[col];
         else if constexpr (EPI == EPI_DTANH)
@@ -254,24 +358,20 @@ namespace hps
   // ls the node's local increment scale, the RMS of the 135 increments plus
   // a floor; x[270] = dt * dt_scale, x[271] = log10(ls * inc_scale) (the
   // local scale relative to the global one, 1 / inc_scale).
-  // One wave per 16 consecutive nodes, 4 lanes per node: each lane handles
-  // a quarter of the stencil (the increment scale is summed over the 4
-  // lanes), the lanes build the node's feature row in LDS, then the wave
-  // writes the 16 rows as contiguous 16-byte stores.
-  constexpr int FEAT_LANES = 4;
-  constexpr int FEAT_NODES = 64 / FEAT_LANES;
+  // A workgroup of 256 lanes handles a tile of 16 x 4 nodes (x, y) of one
+  // z-plane.  The tile's stencil halo (18 x 6 x 3 nodes, both time levels,
+  // clamped at the domain boundary like the stencil) is loaded into LDS
+  // once.  Phase 1, 4 lanes per node: top-2 order parameters and the local
+  // increment scale (sum over the node's 4 lanes in a fixed order).
+  // Phase 2, one lane per (node, stencil point): its 10 features into the
+  // node's row in LDS; the rows then go out as contiguous 16-byte stores.
+  constexpr int FEAT_TX    = 16, FEAT_TY = 4, FEAT_TN = FEAT_TX * FEAT_TY;
+  constexpr int FEAT_HX    = FEAT_TX + 2, FEAT_HY = FEAT_TY + 2;
+  constexpr int FEAT_HALO  = FEAT_HX * FEAT_HY * 3; // halo nodes per time level
+  constexpr int FEAT_CHUNK = FEAT_STRIDE / 8;
 
-  __device__ __forceinline__ void stencil_node(const FeatParams &fp, int gx, int gy, int gz,
-                                               int dx, int dy, int dz, int64_t &nb)
-  {
-    const int xx = min(max(gx + dx, 0), fp.nn[0] - 1);
-    const int yy = min(max(gy + dy, 0), fp.nn[1] - 1);
-    const int zz = min(max(gz + dz, 0), fp.nn[2] - 1);
-    nb           = ((int64_t)zz * fp.nn[1] + yy) * fp.nn[0] + xx;
-  }
-
-  __device__ __forceinline__ void node_fields(const double *__restrict__ a, int G, int i1,
-                                              int i2, double *f)
+  __device__ __forceinline__ void node_fields(const double *a, int G, int i1, int i2,
+                                              double *f)
   {
     double sq = 0.0;
     for (int i = 0; i < G; ++i)
@@ -283,102 +383,170 @@ namespace hps
     f[4] = a[2 + i2];
   }
 
-  __global__ void __launch_bounds__(64)
+  __global__ void __launch_bounds__(256)
     k_features(FeatParams fp, const double *__restrict__ u_prev,
                const double *__restrict__ u_cur, uint16_t *__restrict__ X,
                int *__restrict__ top2, float *__restrict__ lscale)
   {
-    __shared__ __align__(16) uint16_t rows[FEAT_NODES * FEAT_STRIDE];
-    const int     sub  = threadIdx.x % FEAT_LANES; // quarter of the stencil
-    const int     slot = threadIdx.x / FEAT_LANES;
-    const int64_t base = (int64_t)blockIdx.x * FEAT_NODES;
-    const int64_t node = base + slot;
-    const bool    live = node < fp.n_nodes;
-    const int     s0 = (27 * sub) / FEAT_LANES, s1 = (27 * (sub + 1)) / FEAT_LANES;
-    const int     nc = fp.n_comp, G = nc - 2;
-    int           gx = 0, gy = 0, gz = 0, i1 = 0, i2 = 0;
-    double        ss = 0.0;
-    if (live)
+    extern __shared__ __align__(16) unsigned char feat_smem[];
+    uint16_t *rows  = (uint16_t *)feat_smem;                  // [FEAT_TN][FEAT_STRIDE]
+    double   *ls_sh = (double *)(rows + FEAT_TN * FEAT_STRIDE); // [FEAT_TN]
+    int      *i_sh  = (int *)(ls_sh + FEAT_TN);               // [FEAT_TN][2]
+    double   *halo  = (double *)(i_sh + 2 * FEAT_TN);
+    const int nc = fp.n_comp, G = nc - 2;
+    const int nsx = (fp.nn[0] + FEAT_TX - 1) / FEAT_TX;
+    const int nsy = (fp.nn[1] + FEAT_TY - 1) / FEAT_TY;
+    const int bx  = (int)(blockIdx.x % nsx);
+    const int by  = (int)((blockIdx.x / nsx) % nsy);
+    const int gz  = (int)(blockIdx.x / (nsx * nsy));
+    const int x0 = bx * FEAT_TX, y0 = by * FEAT_TY;
+
+    // halo[t][(hz * FEAT_HY + hy) * FEAT_HX + hx][c], t = 0: u_cur, 1: u_prev
+    // (batches of 8 loads per lane in flight before the LDS stores)
+    const int n_load = 2 * FEAT_HALO * nc;
+    for (int i0 = threadIdx.x; i0 < n_load; i0 += 8 * 256)
       {
-        gx = (int)(node % fp.nn[0]);
-        gy = (int)((node / fp.nn[0]) % fp.nn[1]);
-        gz = (int)(node / ((int64_t)fp.nn[0] * fp.nn[1]));
-        // top-2 order parameters at the node at t_n
-        i2        = G > 1 ? 1 : 0;
-        double e1 = -1e300, e2 = -1e300;
-        for (int i = 0; i < G; ++i)
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
           {
-            const double e = u_cur[node * nc + 2 + i];
-            if (e > e1)
+            const int i = i0 + u * 256;
+            v[u]        = 0.0;
+            if (i < n_load)
               {
-                e2 = e1;
-                i2 = i1;
-                e1 = e;
-                i1 = i;
-              }
-            else if (e > e2)
-              {
-                e2 = e;
-                i2 = i;
+                const int c  = i % nc;
+                const int hn = (i / nc) % FEAT_HALO;
+                const int t  = i / (nc * FEAT_HALO);
+                const int hx = hn % FEAT_HX, hy = (hn / FEAT_HX) % FEAT_HY;
+                const int hz = hn / (FEAT_HX * FEAT_HY);
+                const int xx = min(max(x0 + hx - 1, 0), fp.nn[0] - 1);
+                const int yy = min(max(y0 + hy - 1, 0), fp.nn[1] - 1);
+                const int zz = min(max(gz + hz - 1, 0), fp.nn[2] - 1);
+                const int64_t nb = ((int64_t)zz * fp.nn[1] + yy) * fp.nn[0] + xx;
+                v[u]             = (t == 0 ? u_cur : u_prev)[nb * nc + c];
               }
           }
-        if (G == 1)
-          i2 = 0;
-        // pass 1: this lane's part of the sum of squares of the increments
-        for (int s = s0; s < s1; ++s)
-          {
-            int64_t nb;
-            stencil_node(fp, gx, gy, gz, s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1, nb);
-            double fa[5], fb[5];
-            node_fields(u_cur + nb * nc, G, i1, i2, fa);
-            node_fields(u_prev + nb * nc, G, i1, i2, fb);
 #pragma unroll
-            for (int f = 0; f < 5; ++f)
-              {
-                const double d = fa[f] - fb[f];
-                ss += d * d;
-              }
+        for (int u = 0; u < 8; ++u)
+          if (i0 + u * 256 < n_load)
+            halo[i0 + u * 256] = v[u];
+      }
+    __syncthreads();
+    const double *hc = halo;                  // u_cur
+    const double *hp = halo + FEAT_HALO * nc; // u_prev
+    // halo offset of stencil point s of tile node n
+    auto at = [&](int n, int s) {
+      const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
+      return (((1 + dz) * FEAT_HY + (1 + n / FEAT_TX + dy)) * FEAT_HX + 1 + n % FEAT_TX + dx) *
+             nc;
+    };
+
+    // ---- phase 1: 4 lanes per node
+    {
+      const int  sub  = threadIdx.x % 4;
+      const int  n    = threadIdx.x / 4;
+      const int  gx = x0 + n % FEAT_TX, gy = y0 + n / FEAT_TX;
+      const bool live = gx < fp.nn[0] && gy < fp.nn[1];
+      const int  s0 = (27 * sub) / 4, s1 = (27 * (sub + 1)) / 4;
+      int        i1 = 0, i2 = G > 1 ? 1 : 0;
+      double     ss = 0.0;
+      if (live)
+        {
+          const double *ctr = hc + at(n, 13);
+          double        e1 = -1e300, e2 = -1e300;
+          for (int i = 0; i < G; ++i)
+            {
+              const double e = ctr[2 + i];
+              if (e > e1)
+                {
+                  e2 = e1;
+                  i2 = i1;
+                  e1 = e;
+                  i1 = i;
+                }
+              else if (e > e2)
+                {
+                  e2 = e;
+                  i2 = i;
+                }
+            }
+          if (G == 1)
+            i2 = 0;
+          for (int s = s0; s < s1; ++s)
+            {
+              double fa[5], fb[5];
+              node_fields(hc + at(n, s), G, i1, i2, fa);
+              node_fields(hp + at(n, s), G, i1, i2, fb);
+#pragma unroll
+              for (int f = 0; f < 5; ++f)
+                {
+                  const double d = fa[f] - fb[f];
+                  ss += d * d;
+                }
+            }
+        }
+      // fixed order: (q0 + q1) + (q2 + q3)
+      ss += __shfl_xor(ss, 1, 64);
+      ss += __shfl_xor(ss, 2, 64);
+      if (live && sub == 0)
+        {
+          const double  ls   = sqrt(ss / 135.0) + fp.inc_floor;
+          const int64_t node = ((int64_t)gz * fp.nn[1] + gy) * fp.nn[0] + gx;
+          ls_sh[n]           = ls;
+          i_sh[2 * n]        = i1;
+          i_sh[2 * n + 1]    = i2;
+          lscale[node]       = (float)ls;
+          top2[node * 2]     = i1;
+          top2[node * 2 + 1] = i2;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: one lane per (node, stencil point): the point's 10
+    // features into the node's row in LDS
+    for (int it = threadIdx.x; it < FEAT_TN * 27; it += 256)
+      {
+        const int n = it / 27, sp = it % 27;
+        const int gx = x0 + n % FEAT_TX, gy = y0 + n / FEAT_TX;
+        if (gx >= fp.nn[0] || gy >= fp.nn[1])
+          continue;
+        const double inv_ls = 1.0 / ls_sh[n];
+        double       fa[5], fb[5];
+        node_fields(hc + at(n, sp), G, i_sh[2 * n], i_sh[2 * n + 1], fa);
+        node_fields(hp + at(n, sp), G, i_sh[2 * n], i_sh[2 * n + 1], fb);
+        uint16_t *x = rows + n * FEAT_STRIDE + sp * 10;
+#pragma unroll
+        for (int f = 0; f < 5; ++f)
+          {
+            x[f]     = f2bf((float)fa[f]);
+            x[5 + f] = f2bf((float)((fa[f] - fb[f]) * inv_ls));
           }
       }
-    // sum over the node's lanes in a fixed order: (q0 + q1) + (q2 + q3)
-    ss += __shfl_xor(ss, 1, 64);
-    ss += __shfl_xor(ss, 2, 64);
-    if (live)
+    if (threadIdx.x < FEAT_TN)
       {
-        const double ls = sqrt(ss / 135.0) + fp.inc_floor;
-        // pass 2: values and scaled increments of this lane's stencil part
-        uint16_t *x = rows + slot * FEAT_STRIDE;
-        for (int s = s0; s < s1; ++s)
+        const int n = threadIdx.x;
+        if (x0 + n % FEAT_TX < fp.nn[0] && y0 + n / FEAT_TX < fp.nn[1])
           {
-            int64_t nb;
-            stencil_node(fp, gx, gy, gz, s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1, nb);
-            double fa[5], fb[5];
-            node_fields(u_cur + nb * nc, G, i1, i2, fa);
-            node_fields(u_prev + nb * nc, G, i1, i2, fb);
-#pragma unroll
-            for (int f = 0; f < 5; ++f)
-              {
-                x[s * 10 + f]     = f2bf((float)fa[f]);
-                x[s * 10 + 5 + f] = f2bf((float)((fa[f] - fb[f]) / ls));
-              }
-          }
-        if (sub == 0)
-          {
-            x[270]             = f2bf((float)(fp.dt * fp.dt_scale));
-            x[271]             = f2bf((float)log10(ls * fp.inc_scale));
-            lscale[node]       = (float)ls;
-            top2[node * 2]     = i1;
-            top2[node * 2 + 1] = i2;
+            rows[n * FEAT_STRIDE + 270] = f2bf((float)(fp.dt * fp.dt_scale));
+            rows[n * FEAT_STRIDE + 271] = f2bf((float)log10(ls_sh[n] * fp.inc_scale));
           }
       }
     __syncthreads();
-    // write the block's rows: contiguous, 16 bytes per lane and store
-    const int64_t n_rows = min((int64_t)FEAT_NODES, fp.n_nodes - base);
-    const int     n_vec  = (int)(n_rows * FEAT_STRIDE / 8);
-    const uint4  *src    = (const uint4 *)rows;
-    uint4        *dst    = (uint4 *)(X + base * FEAT_STRIDE);
-    for (int i = threadIdx.x; i < n_vec; i += 64)
-      dst[i] = src[i];
+    // ---- write the rows: per tile row of up to 16 nodes, contiguous
+    //      16-byte stores
+    for (int ty = 0; ty < FEAT_TY; ++ty)
+      {
+        const int gy = y0 + ty;
+        if (gy >= fp.nn[1])
+          break;
+        const int     n_rows = min(FEAT_TX, fp.nn[0] - x0);
+        const int     n_vec  = n_rows * FEAT_CHUNK;
+        const int64_t first  = ((int64_t)gz * fp.nn[1] + gy) * fp.nn[0] + x0;
+        const uint4  *src    = (const uint4 *)(rows + ty * FEAT_TX * FEAT_STRIDE);
+        uint4        *dst    = (uint4 *)(X + first * FEAT_STRIDE);
+        for (int i = threadIdx.x; i < n_vec; i += 256)
+          dst[i] = src[i];
+      }
   }
 
   // u_pred = 2 u_n - u_{n-1} + ls * out_scale[k] * y[k] on (c, mu, eta_i1, eta_i2)
@@ -477,8 +645,12 @@ namespace hps
   void mlp_features(hipStream_t s, const FeatParams &fp, const double *u_prev,
                     const double *u_cur, uint16_t *X, int *top2, float *lscale)
   {
-    k_features<<<(int)((fp.n_nodes + FEAT_NODES - 1) / FEAT_NODES), 64, 0, s>>>(
-      fp, u_prev, u_cur, X, top2, lscale);
+    const int    nsx    = (fp.nn[0] + FEAT_TX - 1) / FEAT_TX;
+    const int    nsy    = (fp.nn[1] + FEAT_TY - 1) / FEAT_TY;
+    const size_t shared = FEAT_TN * (FEAT_STRIDE * sizeof(uint16_t) + sizeof(double) +
+                                     2 * sizeof(int)) +
+                          2 * FEAT_HALO * fp.n_comp * sizeof(double);
+    k_features<<<nsx * nsy * fp.nn[2], 256, shared, s>>>(fp, u_prev, u_cur, X, top2, lscale);
   }
   void mlp_predict(hipStream_t s, int64_t n_nodes, int nc, const double *u_prev,
                    const double *u_cur, const float *Y, const int *top2,
