@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--train-iterations", type=int, default=6000)
     ap.add_argument("--q2-cells", type=int, default=96,
                     help="cells per direction of the Q2 tensorial workload (0 = skip)")
+    ap.add_argument("--mp-cells", type=int, default=192,
+                    help="cells per direction of the 27-sphere mixed-precision workload "
+                         "(0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run hps_copy of this many doubles 5 times (calibrates "
                          "the HBM byte counters against a copy of known size) and exit")
@@ -69,6 +72,55 @@ def copy_probe(n):
                              ctypes.c_void_p(x.data_ptr())))
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n}))
+
+
+def mp_vmult(args, timed):
+    """FP64 and FP32 tangent vmult of the mixed-precision workload: 3x3x3
+    packing of 27 spheres (lognormal radii, mean 10, sigma 0.15), Q1 on a
+    uniform mp_cells^3 mesh, 8 order parameters (10 components); per-field
+    relative l2 error of the FP32 vmult (FP32 cache and arithmetic) against
+    the FP64 vmult of the same source"""
+    import torch
+    from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,
+                            SinteringOperatorGeneric, packing)
+    n = args.mp_cells
+    pk = packing.packing_cfg2(seed=0)
+    G = pk.n_order_params
+    mesh = CartesianMesh((n, n, n), pk.lower, pk.upper)
+    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(), MobilityParams())
+    op.set_time_weights(1e3, (-1e3,))
+    u = torch.from_numpy(packing.initial_fields(pk, (n, n, n))).cuda()
+    op.fill_quadrature_point_values(u)
+    src = torch.from_numpy(np.random.default_rng(1234).uniform(-1, 1, op.n_dofs)).cuda()
+    dst = op.initialize_dof_vector()
+    for _ in range(3):
+        op.vmult(dst, src)
+    k = max(5, args.steps // 5)
+    t64 = timed(lambda: op.vmult(dst, src), k)
+    src32, dst32 = src.float(), op.initialize_dof_vector(torch.float32)
+    for _ in range(3):
+        op.vmult(dst32, src32)
+    t32 = timed(lambda: op.vmult(dst32, src32), k)
+    nc = G + 2
+    d64 = dst.view(-1, nc)
+    d32 = dst32.double().view(-1, nc)
+    err = ((d32 - d64).norm(dim=0) / d64.norm(dim=0)).tolist()
+    old = [u.clone()]
+    r = op.initialize_dof_vector()
+    op.evaluate_nonlinear_residual(r, u, 2, old=old)
+    t_res = timed(lambda: op.evaluate_nonlinear_residual(r, u, 2, old=old), k)
+    cells = n ** 3
+    bytes64 = 8 * (2 * op.n_dofs + cells * 8 * (1 + G + 3))
+    out = {"workload": "Q1 %d^3, 27 spheres, %d order parameters (%d components, %d DoF)"
+                       % (n, G, nc, op.n_dofs),
+           "vmult_f64_gdofs": op.n_dofs / t64 / 1e9, "vmult_f64_ms": t64 * 1e3,
+           "vmult_f64_hbm_frac": bytes64 / t64 / 1e9 / HBM_PEAK_GBS,
+           "vmult_f32_gdofs": op.n_dofs / t32 / 1e9,
+           "vmult_f32_rel_l2_error_per_field": err,
+           "residual_f64_gdofs": op.n_dofs / t_res / 1e9}
+    del op, u, src, dst, src32, dst32, r, old
+    torch.cuda.empty_cache()
+    return out
 
 
 def q2_vmult(args, timed):
@@ -101,7 +153,7 @@ def q2_vmult(args, timed):
     for _ in range(3):
         op.vmult(dst32, src32)
     t32 = timed(lambda: op.vmult(dst32, src32), k)
-    nf = 1 + G + (2 + G) * 3
+    nf = 1 + G + 2 * 3 + 1 + 6 + G * 3   # c, eta_i, grad c, grad mu, S, P, Q_i
     cells = n ** 3
     bytes64 = 8 * (2 * op.n_dofs + cells * 27 * nf)
     out = {"workload": "Q2 %d^3, 64 spheres, tensorial mobility, %d components, %d DoF"
@@ -297,6 +349,7 @@ def main():
                          "CPU restatement, not deal.II) on %s" % (reps, _cpu_model())}
 
     q2 = q2_vmult(args, timed) if args.q2_cells > 0 else None
+    mp = mp_vmult(args, timed) if args.mp_cells > 0 else None
 
     steps = None
     if args.timesteps > 0:
@@ -337,6 +390,7 @@ def main():
         "vmult_f32_gdofs": n_dofs / t_vm32 / 1e9,
         "timesteps": steps,
         "q2_tensorial": q2,
+        "mp_27_spheres": mp,
     }
     print(json.dumps(out))
 

@@ -621,15 +621,15 @@ namespace hps
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
               f[CL::grad_mu + d] = g[d];
+            double leg[G][DIM];
 #pragma unroll
             for (int i = 0; i < G; ++i)
               {
-                interp_vg<DIM>(pl[2 + i], gsc, v, g);
+                interp_vg<DIM>(pl[2 + i], gsc, v, leg[i]);
                 f[CL::eta + i] = v;
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  f[CL::grad_eta + i * DIM + d] = g[d];
               }
+            gb_fields<DIM, G>(f + CL::eta, leg, f + CL::grad_mu, f[CL::gb_s], f + CL::gb_p,
+                              f + CL::gb_q);
           }
         const int64_t o = tile_off + (int64_t)k * NF * NT + tid;
 #pragma unroll

@@ -391,15 +391,19 @@ namespace hps
         q2_interp_vg<DIM>(L, U + c * 64, W[0], W[1], W[2], srow, drow, inv_h, v[c], g[c]);
     }
 
+    // tangent: the value rows here; the gradient rows are formed one
+    // component at a time right before that component's integration (each
+    // is a multiple of one input gradient, except the c row), which keeps
+    // fewer values live
     T vr[NC], gr[NC][DIM];
+    T lin[OP == OP_VMULT ? NF : 1];
     if constexpr (OP == OP_VMULT)
       {
-        T        lin[NF];
         const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
 #pragma unroll
         for (int f = 0; f < NF; ++f)
           lin[f] = L.valid ? cl[f * 64] : T(0);
-        tangent_qp<DIM, G, TENS>(kp.pp, lin, v, g, vr, gr);
+        tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
       }
     else
       {
@@ -433,13 +437,30 @@ namespace hps
           q2_s_col(L.t[d], scol[d]);
           q2_dq_col(L.t[d], dcol[d]);
         }
+      // (eta rows first, the c row last: the inputs die in that order)
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
+      for (int cc = 0; cc < NC; ++cc)
         {
+          const int c = NC - 1 - cc;
+          T         gc[DIM];
+          if constexpr (OP == OP_VMULT)
+            {
+              if (c == 0)
+                tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g[0], g[1], gc);
+              else
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  gc[d] = c == 1 ? -T(kp.pp.kappa_c) * g[0][d]
+                                 : T(kp.pp.L) * T(kp.pp.kappa_p) * g[c][d];
+            }
+          else
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              gc[d] = gr[c][d];
           T gq[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; ++d)
-            gq[d] = gr[c][d] * (jxw * inv_h[d]);
+            gq[d] = gc[d] * (jxw * inv_h[d]);
           const T r = q2_integrate<DIM>(L, vr[c] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
           U[c * 64 + L.l] = r;
         }
@@ -475,6 +496,7 @@ namespace hps
         q2_dq_row(L.t[d], drow[d]);
       }
     T f[NF];
+    T leg[TENS ? G : 1][DIM]; // grad eta_i (tensorial mobility)
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       {
@@ -500,9 +522,12 @@ namespace hps
             if constexpr (TENS)
 #pragma unroll
               for (int d = 0; d < DIM; ++d)
-                f[CL::grad_eta + (c - 2) * DIM + d] = g[d];
+                leg[c - 2][d] = g[d];
           }
       }
+    if constexpr (TENS)
+      gb_fields<DIM, G>(f + CL::eta, leg, f + CL::grad_mu, f[CL::gb_s], f + CL::gb_p,
+                        f + CL::gb_q);
     const int64_t o = ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
 #pragma unroll
     for (int q = 0; q < NF; ++q)

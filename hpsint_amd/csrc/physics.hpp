@@ -80,19 +80,82 @@ namespace hps
 
   // number of cached linearisation fields per quadrature point:
   //   scalar mobility:    c, eta_0..eta_{G-1}, grad mu                (1+G+DIM)
-  //   tensorial mobility: c, eta_i, grad c, grad mu, grad eta_i  (1+G+(2+G)DIM)
+  //   tensorial mobility: c, eta_i, grad c, grad mu and the grain-boundary
+  //                       sums S, P, Q_i of gb_fields (1+G+2DIM+1+NP+G DIM)
   // (mu itself and grad c never enter the scalar tangent,
   //  S/mobility.h:477-534, S/operator_sintering_generic.h:104-190)
   template <int DIM, int G, bool TENS>
   struct CacheLayout
   {
-    static constexpr int n_fields = TENS ? 1 + G + (2 + G) * DIM : 1 + G + DIM;
+    static constexpr int NP       = DIM * (DIM + 1) / 2; // symmetric DIM x DIM
+    static constexpr int n_fields = TENS ? 1 + G + 2 * DIM + 1 + NP + G * DIM : 1 + G + DIM;
     static constexpr int c        = 0;
     static constexpr int eta      = 1;
     static constexpr int grad_c   = 1 + G;                    // TENS only
     static constexpr int grad_mu  = TENS ? 1 + G + DIM : 1 + G;
-    static constexpr int grad_eta = 1 + G + 2 * DIM;          // TENS only
+    static constexpr int gb_s     = 1 + G + 2 * DIM;          // TENS only
+    static constexpr int gb_p     = gb_s + 1;                 // TENS only
+    static constexpr int gb_q     = gb_p + NP;                // TENS only
   };
+
+  // index of (a, b), a <= b, in the packed upper triangle of a DIM x DIM
+  // symmetric matrix
+  template <int DIM>
+  __device__ __forceinline__ constexpr int sym_index(int a, int b)
+  {
+    return a * DIM - a * (a - 1) / 2 + (b - a);
+  }
+
+  // Grain-boundary part of MobilityTensorial::apply_M_derivative
+  // (S/mobility.h:809-916): with, over the pairs i > j,
+  //   e_ij = grad eta_i - grad eta_j,  f_ij = unit_vector_filter_2(e_ij),
+  //   S = sum eta_i eta_j,
+  //   P = sum eta_i eta_j f_ij e_ij e_ij^T,
+  //   Q_i = sum_{j != i} eta_j f_ij e_ij (e_ij . grad mu),
+  // the pair sum of the tangent's c row is
+  //   S grad(dmu) - P grad(dmu) + (sum_i deta_i (sum_j eta_j - eta_i)) grad mu
+  //   - sum_i deta_i Q_i,
+  // so the linearisation cache holds S, P, Q_i and the tangent is O(G)
+  // instead of O(G^2) per quadrature point.
+  template <int DIM, int G, typename T>
+  __device__ __forceinline__ void gb_fields(const T *etas, const T (*leg)[DIM], const T *lmu,
+                                            T &S, T *P, T *Q)
+  {
+    constexpr int NP = DIM * (DIM + 1) / 2;
+    S                = T(0);
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+      P[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < G * DIM; ++k)
+      Q[k] = T(0);
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+#pragma unroll
+      for (int j = 0; j < i; ++j)
+        {
+          T egd[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            egd[d] = leg[i][d] - leg[j][d];
+          const T filter = unit_vector_filter_2<DIM>(egd);
+          const T ee     = etas[i] * etas[j];
+          S += ee;
+#pragma unroll
+          for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = a; b < DIM; ++b)
+              P[sym_index<DIM>(a, b)] += ee * filter * egd[a] * egd[b];
+          const T elmu = vdot<DIM>(egd, lmu);
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            {
+              const T r = (filter * egd[d]) * elmu;
+              Q[i * DIM + d] += etas[j] * r;
+              Q[j * DIM + d] += etas[i] * r;
+            }
+        }
+  }
 
   // ------------------------------------------------------------------
   // tangent QP operator: SinteringOperatorGenericQuad::operator()
@@ -193,7 +256,6 @@ namespace hps
       {
         // MobilityTensorial::apply_M_derivative, S/mobility.h:809-916
         const T *lc  = lin + CL::grad_c;
-        const T *leg = lin + CL::grad_eta;
         const T  projector_scale = T(1.001);
         T        nc[DIM];
         unit_vector<DIM>(lc, nc);
@@ -243,34 +305,30 @@ namespace hps
         }
         if constexpr (G > 1)
           {
-            T out_gb[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              out_gb[d] = T(0.0);
+            // the pair sum through the cached S, P, Q_i (gb_fields)
+            const T  S  = lin[CL::gb_s];
+            const T *P  = lin + CL::gb_p;
+            const T *Q  = lin + CL::gb_q;
+            T        eta_sum = T(0.0);
 #pragma unroll
             for (int i = 0; i < G; ++i)
+              eta_sum += etas[i];
+            T ls2 = T(0.0);
 #pragma unroll
-              for (int j = 0; j < i; ++j)
-                {
-                  T egd[DIM];
-#pragma unroll
-                  for (int d = 0; d < DIM; ++d)
-                    egd[d] = leg[i * DIM + d] - leg[j * DIM + d];
-                  const T filter = unit_vector_filter_2<DIM>(egd);
-                  const T emu    = vdot<DIM>(egd, g[1]);
-                  const T ee     = etas[i] * etas[j];
-#pragma unroll
-                  for (int d = 0; d < DIM; ++d)
-                    out_gb[d] += (g[1][d] - egd[d] * (filter * emu)) * ee;
-                  const T elmu = vdot<DIM>(egd, lmu);
-                  const T s2   = etas[j] * v[2 + i] + etas[i] * v[2 + j];
-#pragma unroll
-                  for (int d = 0; d < DIM; ++d)
-                    out_gb[d] += (lmu[d] - (filter * egd[d]) * elmu) * s2;
-                }
+            for (int i = 0; i < G; ++i)
+              ls2 += v[2 + i] * (eta_sum - etas[i]);
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-              out[d] = out[d] + out_gb[d] * (T(2.0) * Mgb);
+              {
+                T o = S * g[1][d] + ls2 * lmu[d];
+#pragma unroll
+                for (int e = 0; e < DIM; ++e)
+                  o -= P[d <= e ? sym_index<DIM>(d, e) : sym_index<DIM>(e, d)] * g[1][e];
+#pragma unroll
+                for (int i = 0; i < G; ++i)
+                  o -= v[2 + i] * Q[i * DIM + d];
+                out[d] = out[d] + o * (T(2.0) * Mgb);
+              }
           }
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
