@@ -808,42 +808,39 @@ namespace hps
     return g > 0 && g < nc && (g % tsz) == 0;
   }
 
-  // one thread per (shared node, component); the three classes of shared
-  // nodes (on a march-direction tile boundary; on a y boundary only; on an x
-  // boundary only) are enumerated back to back, x fastest where possible
-  // (I: index type of the thread decode, int when the thread count fits:
-  // 64-bit divisions are an order of magnitude slower)
-  template <typename T, typename I>
+  // one thread per shared node (all NC components: the partial records of
+  // a node are NC contiguous values, read together); the three classes of
+  // shared nodes (on a march-direction tile boundary; on a y boundary only;
+  // on an x boundary only) are enumerated back to back, x fastest
+  template <typename T, int NC>
   __global__ void __launch_bounds__(256)
     q1_fixup(const FixParams fp, const T *__restrict__ part, T *__restrict__ dst)
   {
-    I         idx = (I)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c   = (int)(idx % fp.n_comp);
-    idx /= fp.n_comp;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
     int g[3];
     if (idx < fp.n_m)
       {
-        g[0] = (int)(idx % fp.nn[0]);
+        g[0] = idx % fp.nn[0];
         idx /= fp.nn[0];
-        g[1] = (int)(idx % fp.nn[1]);
-        g[2] = (int)(idx / fp.nn[1] + 1) * fp.tsz[2];
+        g[1] = idx % fp.nn[1];
+        g[2] = (idx / fp.nn[1] + 1) * fp.tsz[2];
       }
-    else if ((idx -= fp.n_m) < fp.n_y)
+    else if ((idx -= (int)fp.n_m) < fp.n_y)
       {
-        g[0] = (int)(idx % fp.nn[0]);
+        g[0] = idx % fp.nn[0];
         idx /= fp.nn[0];
-        g[2] = (int)(idx % fp.nn[2]);
-        g[1] = (int)(idx / fp.nn[2] + 1) * fp.tsz[1];
+        g[2] = idx % fp.nn[2];
+        g[1] = (idx / fp.nn[2] + 1) * fp.tsz[1];
         if (is_bnd(g[2], fp.tsz[2], fp.nc[2]))
           return;
       }
-    else if ((idx -= fp.n_y) < fp.n_x)
+    else if ((idx -= (int)fp.n_y) < fp.n_x)
       {
-        const I per = fp.nb[0];
-        g[0] = (int)(idx % per + 1) * fp.tsz[0];
+        const int per = fp.nb[0];
+        g[0]          = (idx % per + 1) * fp.tsz[0];
         idx /= per;
-        g[1] = (int)(idx % fp.nn[1]);
-        g[2] = (int)(idx / fp.nn[1]);
+        g[1] = idx % fp.nn[1];
+        g[2] = idx / fp.nn[1];
         if (is_bnd(g[2], fp.tsz[2], fp.nc[2]) || is_bnd(g[1], fp.tsz[1], fp.nc[1]))
           return;
       }
@@ -873,15 +870,24 @@ namespace hps
           }
       }
     const int64_t node = ((int64_t)g[2] * fp.nn[1] + g[1]) * fp.nn[0] + g[0];
-    T             s    = T(0);
+    T             s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      s[c] = T(0);
     for (int a = 0; a < ns[2]; ++a)
       for (int b = 0; b < ns[1]; ++b)
         for (int e = 0; e < ns[0]; ++e)
           {
             const int64_t tile = ((int64_t)tl[2][a] * fp.nt[1] + tl[1][b]) * fp.nt[0] + tl[0][e];
             const int     pn   = lo[0][e] + fp.px * lo[1][b];
-            s += part[((tile * (fp.tsz[2] + 1) + lo[2][a]) * fp.pn + pn) * fp.n_comp + c];
+            const T      *p    = part + ((tile * (fp.tsz[2] + 1) + lo[2][a]) * fp.pn + pn) * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              s[c] += p[c];
           }
-    dst[node * fp.ostride + (int64_t)c * fp.ocstride + fp.ooffset] = s;
+    T *d = dst + node * fp.ostride + fp.ooffset;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      d[(int64_t)c * fp.ocstride] = s[c];
   }
 } // namespace hps

@@ -46,14 +46,29 @@ namespace hps
   template <typename T>
   void launch_fixup(const FixParams &fp, const T *part, T *dst, hipStream_t s)
   {
-    const int64_t n = (fp.n_m + fp.n_y + fp.n_x) * fp.n_comp;
+    const int64_t n = fp.n_m + fp.n_y + fp.n_x; // one thread per shared node
     if (n == 0)
       return;
     const int blocks = (int)((n + 255) / 256);
-    if ((int64_t)blocks * 256 < ((int64_t)1 << 31))
-      q1_fixup<T, int><<<blocks, 256, 0, s>>>(fp, part, dst);
-    else
-      q1_fixup<T, int64_t><<<blocks, 256, 0, s>>>(fp, part, dst);
+    switch (fp.n_comp)
+      {
+#define HPS_CASE(NC)                                                   \
+  case NC:                                                             \
+    q1_fixup<T, NC><<<blocks, 256, 0, s>>>(fp, part, dst);             \
+    break;
+        HPS_CASE(3)
+        HPS_CASE(4)
+        HPS_CASE(5)
+        HPS_CASE(6)
+        HPS_CASE(7)
+        HPS_CASE(8)
+        HPS_CASE(10)
+        HPS_CASE(12)
+        HPS_CASE(14)
+#undef HPS_CASE
+        default:
+          break;
+      }
   }
   template void launch_fixup<double>(const FixParams &, const double *, double *, hipStream_t);
   template void launch_fixup<float>(const FixParams &, const float *, float *, hipStream_t);
