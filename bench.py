@@ -32,6 +32,11 @@ HBM_PEAK_GBS = 8000.0
 # scaled by the same counter's reading for a copy of known size; counts and
 # factors in profiles/vmult_q1_128cubed_hbm_traffic.md
 VMULT_TRAFFIC_128 = 1.3841e9
+# FP64 flop per launch of the residual main kernel on the same workload
+# (SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes, FMA = 2;
+# profiles/q1_kernels_128cubed_fp64_counters.md); FP64 vector peak
+RESIDUAL_FLOP_128 = 8.35e9
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
@@ -387,6 +392,8 @@ def main():
         "bytes_per_vmult_reference_cache": bytes_vmult_ref,
         "residual_gdofs": n_dofs / t_res / 1e9,
         "residual_hbm_frac": res_achieved / HBM_PEAK_GBS,
+        "residual_fp64_frac": (RESIDUAL_FLOP_128 / (float(np.mean(res_main)) * 1e-3) / 1e12 /
+                               FP64_PEAK_TFLOPS) if (n == 128 and G == 4) else None,
         "vmult_f32_gdofs": n_dofs / t_vm32 / 1e9,
         "timesteps": steps,
         "q2_tensorial": q2,
