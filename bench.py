@@ -57,6 +57,14 @@ def parse():
     ap.add_argument("--mp-cells", type=int, default=192,
                     help="cells per direction of the 27-sphere mixed-precision workload "
                          "(0 = skip)")
+    ap.add_argument("--predictor-study", action="store_true",
+                    help="only run the held-out predictor study on the 100-sphere "
+                         "packings (12 order parameters) and exit")
+    ap.add_argument("--study-cells", type=int, default=160)
+    ap.add_argument("--study-seeds", type=int, default=4,
+                    help="packings (seeds 0..k-1) whose plain trajectories train")
+    ap.add_argument("--study-train-steps", type=int, default=12)
+    ap.add_argument("--study-test-steps", type=int, default=5)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run hps_copy of this many doubles 5 times (calibrates "
                          "the HBM byte counters against a copy of known size) and exit")
@@ -77,6 +85,69 @@ def copy_probe(n):
                              ctypes.c_void_p(x.data_ptr())))
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n}))
+
+
+def predictor_study(args):
+    """Learned predictor on held-out packings: plain FP64 trajectories of
+    packings with seeds 0..k-1 (random sequential packings of 100 spheres,
+    lognormal radii mean 8, sigma 0.2, 12 order parameters, Q1 on a
+    study_cells^3 mesh, BDF1 dt = 1e-3) train one network; on the packing
+    with seed k, after 3 plain steps, study_test_steps steps are timed per
+    method at Newton rtol 1e-8."""
+    import torch
+    from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,
+                            SinteringOperatorGeneric, packing)
+    from hpsint_amd.predictor_training import collect_trajectory, fit_predictor
+    from hpsint_amd.solvers import NewtonSolver, TimeStepper
+    n = args.study_cells
+    G = 12
+    dt = 1e-3
+    pk0 = packing.packing_cfg4(seed=0)
+    mesh = CartesianMesh((n, n, n), pk0.lower, pk0.upper)
+    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(), MobilityParams())
+
+    def fields(seed):
+        pk = packing.packing_cfg4(seed=seed)
+        assert pk.n_order_params <= G
+        return torch.from_numpy(packing.initial_fields(pk, (n, n, n), n_order_params=G)).cuda()
+
+    t0 = time.perf_counter()
+    trajs = [collect_trajectory(op, fields(s), n_steps=args.study_train_steps, dt=dt,
+                                nl_rel_tol=1e-8) for s in range(args.study_seeds)]
+    t_traj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pred = fit_predictor(op, trajs, dt, iterations=args.train_iterations,
+                         rows_per_step=min(131072, op.n_nodes), seed=0,
+                         column_weights=(3.0, 1.0, 1.0, 1.0))
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    del trajs
+    torch.cuda.empty_cache()
+    held = collect_trajectory(op, fields(args.study_seeds), n_steps=3, dt=dt, nl_rel_tol=1e-8)
+
+    def run(fp32, predictor):
+        solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
+                              preconditioner="InverseComponentBlockDiagonalMatrix",
+                              inner_fp32=fp32)
+        s = TimeStepper(op, solver, order=1, dt=dt, predictor=predictor)
+        s.initialize(held[3])
+        s.history = [held[3].clone(), held[2].clone()]
+        s.t = 3 * dt
+        rs = [s.make_step() for _ in range(args.study_test_steps)]
+        return {"s_per_step": sum(r.seconds for r in rs) / len(rs),
+                "newton": [r.newton_iterations for r in rs],
+                "linear": [r.linear_iterations for r in rs]}
+
+    out = {"workload": "100 spheres, Q1 %d^3, %d order parameters (%d DoF); trained on %d "
+                       "packings x %d plain steps, tested on a held-out packing"
+                       % (n, G, op.n_dofs, args.study_seeds, args.study_train_steps),
+           "fp64_plain": run(False, None), "mp_learned": run(True, pred),
+           "mp_only": run(True, None), "fp64_learned": run(False, pred),
+           "trajectory_s": t_traj, "predictor_training_s": t_fit,
+           "predictor_final_loss": pred.losses[-1]}
+    out["speedup_mp_learned_vs_fp64"] = (out["fp64_plain"]["s_per_step"] /
+                                         out["mp_learned"]["s_per_step"])
+    return out
 
 
 def mp_vmult(args, timed):
@@ -246,6 +317,9 @@ def main():
         sys.exit(2)
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return
+    if args.predictor_study:
+        print(json.dumps({"predictor_study": predictor_study(args)}))
         return
 
     import torch

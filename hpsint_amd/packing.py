@@ -169,6 +169,15 @@ def packing_cfg3(seed=0):
     return p
 
 
+def packing_cfg4(seed=0):
+    """random sequential packing of 100 spheres, lognormal radii (mean 8,
+    sigma 0.2) in a 160^3 box; interaction buffer 2 max radius: 10-12 order
+    parameters for seeds 0-4 (the operator is built with 12)"""
+    p = random_packing(100, 8.0, 0.2, 160.0, seed=seed)
+    p.colors = greedy_coloring(interaction_graph(p, radius_buffer_ratio=2.0))
+    return p
+
+
 def initial_fields(p: Packing, n_cells, w=1.0, n_order_params=None, dtype=np.float64):
     """interleaved nodal state (node-major, x fastest): c, mu, eta_0.."""
     dim = p.dim

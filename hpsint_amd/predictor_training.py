@@ -33,9 +33,10 @@ def collect_trajectory(op, u0, n_steps, dt, nl_rel_tol=1e-8, l_rel_tol=1e-2,
 
 def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0,
                   column_weights=(1.0, 1.0, 1.0, 1.0)):
-    """rows of (features, targets) from every triple of traj; sets
-    pred.out_scale to the RMS of each target column divided by its weight
-    and normalises by it (a column of weight w counts w^2 in the loss)"""
+    """rows of (features, targets) from every triple of traj (one trajectory,
+    or a list of trajectories of the same mesh); sets pred.out_scale to the
+    RMS of each target column divided by its weight and normalises by it (a
+    column of weight w counts w^2 in the loss)"""
     op = pred.op
     n = op.n_nodes
     dev = pred.X.device
@@ -43,9 +44,11 @@ def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0,
     pred.out_scale.fill_(1.0)
     Xs, Ts = [], []
     T = torch.empty(n, N_OUT, dtype=torch.float32, device=dev)
-    for i in range(1, len(traj) - 1):
-        X, _ = pred.features(traj[i - 1], traj[i], dt)
-        pred.targets(traj[i - 1], traj[i], traj[i + 1], T)
+    trajs = traj if isinstance(traj[0], (list, tuple)) else [traj]
+    triples = [(t[i - 1], t[i], t[i + 1]) for t in trajs for i in range(1, len(t) - 1)]
+    for (u_prev, u_cur, u_next) in triples:
+        X, _ = pred.features(u_prev, u_cur, dt)
+        pred.targets(u_prev, u_cur, u_next, T)
         k = min(rows_per_step // 2, n)
         act = T.abs().sum(dim=1)
         top = torch.topk(act, k).indices
@@ -64,11 +67,15 @@ def build_dataset(pred: LearnedPredictor, traj, dt, rows_per_step, seed=0,
 def fit_predictor(op, traj, dt, iterations=1000, rows_per_step=32768, batch=16384,
                   lr=1e-3, seed=0, log=None, skip=2, column_weights=(1.0, 1.0, 1.0, 1.0)):
     """trains on the triples of traj[skip:] (the first steps of a run are the
-    relaxation of the initial condition and are left out).  column_weights
+    relaxation of the initial condition and are left out); traj may be a list
+    of trajectories (several packings on one mesh).  column_weights
     weight the outputs (c, mu, eta_top1, eta_top2) in the loss: the c
     error enters the residual through the mass term w0 = 1/dt and dominates
     it, so weighting c up lowers the residual of the prediction."""
-    traj = traj[skip:]
+    if isinstance(traj[0], (list, tuple)):
+        traj = [t[skip:] for t in traj]
+    else:
+        traj = traj[skip:]
     mlp = MLP(op, seed=seed)
     pred = LearnedPredictor(op, mlp, dt_scale=1.0 / dt)
     X, T = build_dataset(pred, traj, dt, rows_per_step, seed, column_weights)

@@ -31,6 +31,8 @@ def main():
            "cfg1_8_spheres": dump(packing.packing_cfg1(seed=0)),
            "cfg2_27_spheres": dump(packing.packing_cfg2(seed=0)),
            "cfg3_64_spheres": dump(packing.packing_cfg3(seed=0))}
+    for seed in range(5):
+        out["study_100_spheres_seed%d" % seed] = dump(packing.packing_cfg4(seed=seed))
     with open(os.path.join(HERE, "packings.json"), "w") as f:
         json.dump(out, f, indent=1)
 

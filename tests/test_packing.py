@@ -16,9 +16,12 @@ GEN = {"two_particles_2d": packing.two_particles_2d,
        "cfg1_8_spheres": lambda: packing.packing_cfg1(seed=0),
        "cfg2_27_spheres": lambda: packing.packing_cfg2(seed=0),
        "cfg3_64_spheres": lambda: packing.packing_cfg3(seed=0)}
+for _s in range(5):
+    GEN["study_100_spheres_seed%d" % _s] = (lambda s: lambda: packing.packing_cfg4(seed=s))(_s)
 
 BUFFER = {"two_particles_2d": 0.0, "cfg1_8_spheres": 0.75, "cfg2_27_spheres": 1.25,
           "cfg3_64_spheres": 2.075}
+BUFFER.update({"study_100_spheres_seed%d" % s: 2.0 for s in range(5)})
 
 
 @pytest.mark.parametrize("name", sorted(GEN))
@@ -48,3 +51,5 @@ def test_config_order_parameter_counts():
     assert packing.packing_cfg2(seed=0).n_order_params == 8
     assert packing.packing_cfg3(seed=0).n_order_params == 10
     assert packing.two_particles_2d().n_order_params == 2
+    # the predictor study builds one operator with 12 order parameters
+    assert max(packing.packing_cfg4(seed=s).n_order_params for s in range(5)) <= 12
