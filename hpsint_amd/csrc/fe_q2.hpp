@@ -347,6 +347,85 @@ namespace hps
       }
   }
 
+  // tangent vmult of the workgroup's cells: U holds the gathered increment
+  // on entry and the integrated rows on exit.  Only the gradients of c and
+  // mu are interpolated with the values; the cache is read once, for the
+  // value rows and the c gradient row, and dies before the integrations;
+  // the gradient of eta_i is formed from its Gauss-point values right
+  // before row i is integrated (it enters that row only, through
+  // L kappa_p grad eta_i), so few values stay live per lane.
+  template <int DIM, int G, bool TENS, typename T>
+  __device__ __forceinline__ void q2_vmult_body(const Q2Params &kp, const Q2Lane<DIM> &L, T *U,
+                                                T (*W)[64], const T *__restrict__ cache,
+                                                const T *inv_h)
+  {
+    using CL         = CacheLayout<DIM, G, TENS>;
+    constexpr int NC = G + 2;
+    constexpr int NF = CL::n_fields;
+    T             v[NC], g0[DIM], g1[DIM];
+    {
+      T srow[DIM][3], drow[DIM][3];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        {
+          q2_s_row(L.t[d], srow[d]);
+          q2_dq_row(L.t[d], drow[d]);
+        }
+      q2_interp_vg<DIM>(L, U, W[0], W[1], W[2], srow, drow, inv_h, v[0], g0);
+      q2_interp_vg<DIM>(L, U + 64, W[0], W[1], W[2], srow, drow, inv_h, v[1], g1);
+#pragma unroll
+      for (int c = 2; c < NC; ++c)
+        v[c] = q2_interp_v<DIM>(L, U + c * 64, W[0], W[1], srow);
+    }
+    T vr[NC], gc0[DIM];
+    {
+      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
+      T        lin[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        lin[f] = L.valid ? cl[f * 64] : T(0);
+      tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+      tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0);
+    }
+    T jxw = T(kp.h_prod);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      jxw *= q2_w<T>(L.t[d]);
+    T scol[DIM][3], dcol[DIM][3], drow[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        q2_s_col(L.t[d], scol[d]);
+        q2_dq_col(L.t[d], dcol[d]);
+        q2_dq_row(L.t[d], drow[d]);
+      }
+    T gq[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      gq[d] = gc0[d] * (jxw * inv_h[d]);
+    U[L.l] = q2_integrate<DIM>(L, vr[0] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+    const T kc = -T(kp.pp.kappa_c);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      gq[d] = kc * g0[d] * (jxw * inv_h[d]);
+    U[64 + L.l] = q2_integrate<DIM>(L, vr[1] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+    const T lkp = T(kp.pp.L) * T(kp.pp.kappa_p) * jxw;
+#pragma unroll
+    for (int c = 2; c < NC; ++c)
+      {
+        // (W[3] is free here: q2_integrate last read it before its final
+        // barrier and writes it only after its first one)
+        W[3][L.l] = v[c];
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+          gq[d] = q2_pass(W[3], L.base(d), Q2Lane<DIM>::str(d), drow[d]) *
+                  (lkp * inv_h[d] * inv_h[d]);
+        U[c * 64 + L.l] =
+          q2_integrate<DIM>(L, vr[c] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+      }
+  }
+
   // ----------------------------------------- tangent vmult and residual
   template <int DIM, int G, bool TENS, int OP, typename T>
   __global__ void __launch_bounds__(64)
@@ -377,7 +456,14 @@ namespace hps
     __syncthreads();
 
     const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
-    T       v[NC], g[NC][DIM];
+    if constexpr (OP == OP_VMULT)
+      {
+        q2_vmult_body<DIM, G, TENS>(kp, L, U, W, cache, inv_h);
+        __syncthreads();
+        q2_scatter<DIM, NC, T>(kp, U, dst, NC, 1, 0);
+        return;
+      }
+    T v[NC], g[NC][DIM];
     {
       T srow[DIM][3], drow[DIM][3];
 #pragma unroll
