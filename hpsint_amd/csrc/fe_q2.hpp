@@ -307,22 +307,73 @@ namespace hps
     return r;
   }
 
+  // the workgroup's cells (the same for every lane): node index of local
+  // node 0 of each cell slot (-1: no cell) and the cell coordinates, so that
+  // the per-item index work of a gather or scatter is a few multiply-adds
+  template <int DIM>
+  struct Q2Cells
+  {
+    using LN = Q2Lane<DIM>;
+    int64_t n0[LN::CPW];
+    int     cc[LN::CPW][3];
+    int64_t sy, sz;
+
+    __device__ explicit Q2Cells(const Q2Params &kp)
+    {
+      sy = kp.nn[0];
+      sz = (int64_t)kp.nn[0] * kp.nn[1];
+#pragma unroll
+      for (int k = 0; k < LN::CPW; ++k)
+        {
+          int c[3] = {0, 0, 0};
+          n0[k]    = LN::cell(kp, (int64_t)blockIdx.x * LN::CPW + k, c) ? LN::node(kp, c, 0) : -1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            cc[k][d] = c[d];
+        }
+    }
+    // node of local node p of cell slot s (-1: no cell); c: the cell
+    __device__ int64_t node(int s, int p, int *c) const
+    {
+      int64_t b = -1;
+      c[0] = c[1] = c[2] = 0;
+#pragma unroll
+      for (int k = 0; k < LN::CPW; ++k)
+        if (s == k)
+          {
+            b    = n0[k];
+            c[0] = cc[k][0];
+            c[1] = cc[k][1];
+            c[2] = cc[k][2];
+          }
+      const int tx = p % 3, ty = (p / 3) % 3, tz = DIM == 3 ? p / 9 : 0;
+      return b < 0 ? int64_t(-1) : b + tx + ty * sy + tz * sz;
+    }
+  };
+
   // gather n_comp components of the cells of this workgroup into
   // buf[c * 64 + lane] (lane order); missing cells read as zero
   template <int DIM, int NC, typename T, typename F>
   __device__ __forceinline__ void q2_gather(const Q2Params &kp, T *buf, F &&value)
   {
-    using LN       = Q2Lane<DIM>;
+    using LN         = Q2Lane<DIM>;
     constexpr int NI = LN::CPW * LN::NP * NC;
-    for (int f = threadIdx.x; f < NI; f += 64)
+    const Q2Cells<DIM> wc(kp);
+#pragma unroll
+    for (int it = 0; it < (NI + 63) / 64; ++it)
       {
-        const int sp = f / NC, c = f - sp * NC;
-        const int s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
-        int       cc[3];
-        T         v = T(0);
-        if (LN::cell(kp, (int64_t)blockIdx.x * LN::CPW + s2, cc))
-          v = value(LN::node(kp, cc, p2) * NC + c);
-        buf[c * 64 + sp] = v;
+        const int f = it * 64 + (int)threadIdx.x;
+        if (f < NI)
+          {
+            const int     sp = f / NC, c = f - sp * NC;
+            const int     s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
+            int           cc[3];
+            const int64_t n = wc.node(s2, p2, cc);
+            T             v = T(0);
+            if (n >= 0)
+              v = value(n * NC + c);
+            buf[c * 64 + sp] = v;
+          }
       }
   }
 
@@ -332,37 +383,47 @@ namespace hps
   __device__ __forceinline__ void q2_scatter(const Q2Params &kp, const T *buf, T *dst, int os,
                                              int ocs, int oo)
   {
-    using LN       = Q2Lane<DIM>;
+    using LN         = Q2Lane<DIM>;
     constexpr int NI = LN::CPW * LN::NP * NC;
-    for (int f = threadIdx.x; f < NI; f += 64)
+    const Q2Cells<DIM> wc(kp);
+#pragma unroll
+    for (int it = 0; it < (NI + 63) / 64; ++it)
       {
-        const int sp = f / NC, c = f - sp * NC;
-        const int s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
-        int       cc[3];
-        if (!LN::cell(kp, (int64_t)blockIdx.x * LN::CPW + s2, cc))
-          continue;
-        T      *d = dst + LN::node(kp, cc, p2) * os + (int64_t)c * ocs + oo;
-        const T r = buf[c * 64 + sp];
-        *d        = LN::first(kp, cc, p2) ? r : *d + r;
+        const int f = it * 64 + (int)threadIdx.x;
+        if (f < NI)
+          {
+            const int     sp = f / NC, c = f - sp * NC;
+            const int     s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
+            int           cc[3];
+            const int64_t n = wc.node(s2, p2, cc);
+            if (n >= 0)
+              {
+                T      *d = dst + n * os + (int64_t)c * ocs + oo;
+                const T r = buf[c * 64 + sp];
+                *d        = LN::first(kp, cc, p2) ? r : *d + r;
+              }
+          }
       }
   }
 
   // tangent vmult of the workgroup's cells: U holds the gathered increment
-  // on entry and the integrated rows on exit.  Only the gradients of c and
-  // mu are interpolated with the values; the cache is read once, for the
-  // value rows and the c gradient row, and dies before the integrations;
-  // the gradient of eta_i is formed from its Gauss-point values right
-  // before row i is integrated (it enters that row only, through
-  // L kappa_p grad eta_i), so few values stay live per lane.
+  // on entry and the integrated rows on exit; VR (NC rows of 64) takes the
+  // value rows.  Only the gradients of c and mu are interpolated with the
+  // values; the Gauss-point values of eta_i replace their nodal values in U
+  // and the gradient of eta_i is formed from them right before row i is
+  // integrated (it enters that row only, through L kappa_p grad eta_i).  The
+  // cache is read once, for the value rows and the c gradient row.  Few
+  // values stay live per lane.
   template <int DIM, int G, bool TENS, typename T>
   __device__ __forceinline__ void q2_vmult_body(const Q2Params &kp, const Q2Lane<DIM> &L, T *U,
-                                                T (*W)[64], const T *__restrict__ cache,
+                                                T *VR, T (*W)[64], const T *__restrict__ cache,
                                                 const T *inv_h)
   {
     using CL         = CacheLayout<DIM, G, TENS>;
     constexpr int NC = G + 2;
     constexpr int NF = CL::n_fields;
-    T             v[NC], g0[DIM], g1[DIM];
+    const int     l  = L.l;
+    T             v0, v1, g0[DIM], g1[DIM];
     {
       T srow[DIM][3], drow[DIM][3];
 #pragma unroll
@@ -371,58 +432,101 @@ namespace hps
           q2_s_row(L.t[d], srow[d]);
           q2_dq_row(L.t[d], drow[d]);
         }
-      q2_interp_vg<DIM>(L, U, W[0], W[1], W[2], srow, drow, inv_h, v[0], g0);
-      q2_interp_vg<DIM>(L, U + 64, W[0], W[1], W[2], srow, drow, inv_h, v[1], g1);
+      q2_interp_vg<DIM>(L, U, W[0], W[1], W[2], srow, drow, inv_h, v0, g0);
+      q2_interp_vg<DIM>(L, U + 64, W[0], W[1], W[2], srow, drow, inv_h, v1, g1);
+      // (q2_interp_v ends after a barrier that follows every read of the
+      // nodal row, so the row can take the Gauss-point values)
 #pragma unroll
       for (int c = 2; c < NC; ++c)
-        v[c] = q2_interp_v<DIM>(L, U + c * 64, W[0], W[1], srow);
-    }
-    T vr[NC], gc0[DIM];
-    {
-      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
-      T        lin[NF];
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-        lin[f] = L.valid ? cl[f * 64] : T(0);
-      tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
-      tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0);
+        {
+          const T x    = q2_interp_v<DIM>(L, U + c * 64, W[0], W[1], srow);
+          U[c * 64 + l] = x;
+        }
     }
     T jxw = T(kp.h_prod);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       jxw *= q2_w<T>(L.t[d]);
-    T scol[DIM][3], dcol[DIM][3], drow[DIM][3];
+    T        gc0[DIM];
+    const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + l;
+    {
+      T v[NC], vr[NC];
+      v[0] = v0;
+      v[1] = v1;
+#pragma unroll
+      for (int c = 2; c < NC; ++c)
+        v[c] = U[c * 64 + l];
+      if constexpr (TENS && G > 1)
+        {
+          // sum_i deta_i Q_i streamed from the cache first; the barrier
+          // keeps the loads of the other fields below it
+          T qsum[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            qsum[d] = T(0);
+#pragma unroll
+          for (int i = 0; i < G; ++i)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              qsum[d] += v[2 + i] * (L.valid ? cl[(CL::gb_q + i * DIM + d) * 64] : T(0));
+          __syncthreads();
+          T lin[CL::gb_q];
+#pragma unroll
+          for (int f = 0; f < CL::gb_q; ++f)
+            lin[f] = L.valid ? cl[f * 64] : T(0);
+          tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+          tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0, qsum);
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            VR[c * 64 + l] = vr[c] * jxw;
+        }
+      else
+        {
+          T lin[NF];
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            lin[f] = L.valid ? cl[f * 64] : T(0);
+          tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+          tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0);
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            VR[c * 64 + l] = vr[c] * jxw;
+        }
+    }
+    T scol[DIM][3], dcol[DIM][3];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       {
         q2_s_col(L.t[d], scol[d]);
         q2_dq_col(L.t[d], dcol[d]);
-        q2_dq_row(L.t[d], drow[d]);
       }
     T gq[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       gq[d] = gc0[d] * (jxw * inv_h[d]);
-    U[L.l] = q2_integrate<DIM>(L, vr[0] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+    U[l] = q2_integrate<DIM>(L, VR[l], gq, W[0], W[1], W[2], W[3], scol, dcol);
     const T kc = -T(kp.pp.kappa_c);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       gq[d] = kc * g0[d] * (jxw * inv_h[d]);
-    U[64 + L.l] = q2_integrate<DIM>(L, vr[1] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+    U[64 + l] = q2_integrate<DIM>(L, VR[64 + l], gq, W[0], W[1], W[2], W[3], scol, dcol);
     const T lkp = T(kp.pp.L) * T(kp.pp.kappa_p) * jxw;
-#pragma unroll
+#pragma unroll 1
     for (int c = 2; c < NC; ++c)
       {
-        // (W[3] is free here: q2_integrate last read it before its final
-        // barrier and writes it only after its first one)
-        W[3][L.l] = v[c];
-        __syncthreads();
+        // gradient of eta_c from its Gauss-point values in row c (written
+        // before earlier barriers; read before the first barrier of
+        // q2_integrate, whose result replaces the row after its last one)
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
-          gq[d] = q2_pass(W[3], L.base(d), Q2Lane<DIM>::str(d), drow[d]) *
-                  (lkp * inv_h[d] * inv_h[d]);
-        U[c * 64 + L.l] =
-          q2_integrate<DIM>(L, vr[c] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
+          {
+            T dr[3];
+            q2_dq_row(L.t[d], dr);
+            gq[d] = q2_pass(U + c * 64, L.base(d), Q2Lane<DIM>::str(d), dr) *
+                    (lkp * inv_h[d] * inv_h[d]);
+          }
+        U[c * 64 + l] =
+          q2_integrate<DIM>(L, VR[c * 64 + l], gq, W[0], W[1], W[2], W[3], scol, dcol);
       }
   }
 
@@ -439,7 +543,7 @@ namespace hps
     constexpr bool HIST = OP == OP_RES2;
 
     __shared__ T U[NC * 64];
-    __shared__ T H[HIST ? NC * 64 : 1];
+    __shared__ T H[(HIST || OP == OP_VMULT) ? NC * 64 : 1];
     __shared__ T W[4][64];
 
     const Q2Lane<DIM> L(kp);
@@ -458,7 +562,7 @@ namespace hps
     const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
     if constexpr (OP == OP_VMULT)
       {
-        q2_vmult_body<DIM, G, TENS>(kp, L, U, W, cache, inv_h);
+        q2_vmult_body<DIM, G, TENS>(kp, L, U, H, W, cache, inv_h);
         __syncthreads();
         q2_scatter<DIM, NC, T>(kp, U, dst, NC, 1, 0);
         return;

@@ -212,13 +212,16 @@ namespace hps
     s = dMdc * v[0] + (T(2.0) * Mgb) * factor;
   }
 
+  // qsum (optional, tensorial only): sum_i deta_i Q_i formed by the caller,
+  // which then need not hold the Q_i fields of lin
   template <int DIM, int G, bool TENS, bool VALS, bool GRADC, typename T>
   __device__ __forceinline__ void tangent_qp_rows(const PhysParams &pp,
                                                   const T          *lin,
                                                   const T          *v,
                                                   const T (*g)[DIM],
                                                   T                *vr,
-                                                  T (*gr)[DIM])
+                                                  T (*gr)[DIM],
+                                                  const T          *qsum = nullptr)
   {
     using CL        = CacheLayout<DIM, G, TENS>;
     const T  c      = lin[CL::c];
@@ -324,9 +327,12 @@ namespace hps
 #pragma unroll
                 for (int e = 0; e < DIM; ++e)
                   o -= P[d <= e ? sym_index<DIM>(d, e) : sym_index<DIM>(e, d)] * g[1][e];
+                if (qsum)
+                  o -= qsum[d];
+                else
 #pragma unroll
-                for (int i = 0; i < G; ++i)
-                  o -= v[2 + i] * Q[i * DIM + d];
+                  for (int i = 0; i < G; ++i)
+                    o -= v[2 + i] * Q[i * DIM + d];
                 out[d] = out[d] + o * (T(2.0) * Mgb);
               }
           }
@@ -413,7 +419,7 @@ namespace hps
   template <int DIM, int G, bool TENS, typename T>
   __device__ __forceinline__ void tangent_grad_c(const PhysParams &pp, const T *lin,
                                                  const T *v, const T *g0, const T *g1,
-                                                 T *out)
+                                                 T *out, const T *qsum = nullptr)
   {
     T g[G + 2][DIM], vr[G + 2], gr[G + 2][DIM];
 #pragma unroll
@@ -427,7 +433,7 @@ namespace hps
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
         g[c][d] = T(0);
-    tangent_qp_rows<DIM, G, TENS, false, true>(pp, lin, v, g, vr, gr);
+    tangent_qp_rows<DIM, G, TENS, false, true>(pp, lin, v, g, vr, gr, qsum);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       out[d] = gr[0][d];
