@@ -6,7 +6,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/hpsint_amd.h"
 #include "blas.hpp"
@@ -83,6 +85,9 @@ struct hps_operator_s
   hipEvent_t           ev[3];
   bool                 timing;
   const double        *old[3];
+  int64_t              n_constrained;    // constrained nodes (none by default)
+  int64_t             *constrained;      // their node indices (device)
+  double              *constrained_save; // n_constrained * nc saved src entries
 };
 
 static int check_op(hps_operator op)
@@ -297,6 +302,8 @@ extern "C" int hps_destroy(hps_operator op)
   (void)hipFree(op->inv_blocks64);
   (void)hipFree(op->inv_blocks32);
   (void)hipFree(op->d_singular);
+  (void)hipFree(op->constrained);
+  (void)hipFree(op->constrained_save);
   for (int i = 0; i < 3; ++i)
     if (op->ev[i])
       (void)hipEventDestroy(op->ev[i]);
@@ -417,6 +424,43 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+  // FEEvaluation::distribute_local_to_global leaves constrained entries
+  // untouched: their residual rows are zero
+  if (op->n_constrained)
+    {
+      hps::constraint_set(op->stream, op->n_constrained, op->nc, op->constrained, dst, 0.0);
+      HPS_LAUNCHED();
+    }
+  return HPS_OK;
+}
+
+extern "C" int hps_set_constrained_nodes(hps_operator op, const int64_t *nodes, int64_t n)
+{
+  if (int r = check_op(op))
+    return r;
+  if (n < 0 || (n > 0 && !nodes))
+    return fail(HPS_ERR_INVALID_ARGUMENT, "constrained nodes: bad arguments");
+  std::vector<int64_t> v(nodes, nodes + n);
+  std::sort(v.begin(), v.end());
+  for (int64_t i = 0; i < n; ++i)
+    {
+      if (v[i] < 0 || v[i] >= op->n_nodes)
+        return fail(HPS_ERR_INVALID_ARGUMENT, "constrained node index out of range");
+      if (i > 0 && v[i] == v[i - 1])
+        return fail(HPS_ERR_INVALID_ARGUMENT, "constrained node listed twice");
+    }
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  (void)hipFree(op->constrained);
+  (void)hipFree(op->constrained_save);
+  op->constrained      = nullptr;
+  op->constrained_save = nullptr;
+  op->n_constrained    = 0;
+  if (n == 0)
+    return HPS_OK;
+  HPS_HIP(hipMalloc(&op->constrained, n * sizeof(int64_t)));
+  HPS_HIP(hipMalloc(&op->constrained_save, n * op->nc * sizeof(double)));
+  HPS_HIP(hipMemcpy(op->constrained, v.data(), n * sizeof(int64_t), hipMemcpyHostToDevice));
+  op->n_constrained = n;
   return HPS_OK;
 }
 
@@ -428,6 +472,14 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
     return fail(HPS_ERR_INVALID_ARGUMENT, "vmult: null vector");
   if (!op->lin_set)
     return fail(HPS_ERR_NOT_INITIALIZED, "vmult before hps_set_linearization");
+  // constrained entries of src: saved and zeroed, restored after the
+  // operator (src is modified in between, in stream order)
+  if (op->n_constrained)
+    {
+      hps::constraint_zero64(op->stream, op->n_constrained, op->nc, op->constrained,
+                             const_cast<double *>(src), op->constrained_save);
+      HPS_LAUNCHED();
+    }
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
   if (op->q2)
@@ -441,6 +493,12 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+  if (op->n_constrained)
+    {
+      hps::constraint_restore64(op->stream, op->n_constrained, op->nc, op->constrained,
+                                const_cast<double *>(src), dst, op->constrained_save);
+      HPS_LAUNCHED();
+    }
   return HPS_OK;
 }
 
@@ -452,6 +510,13 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
     return fail(HPS_ERR_INVALID_ARGUMENT, "vmult: null vector");
   if (!op->lin_set)
     return fail(HPS_ERR_NOT_INITIALIZED, "vmult before hps_set_linearization");
+  float *save32 = reinterpret_cast<float *>(op->constrained_save);
+  if (op->n_constrained)
+    {
+      hps::constraint_zero32(op->stream, op->n_constrained, op->nc, op->constrained,
+                             const_cast<float *>(src), save32);
+      HPS_LAUNCHED();
+    }
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
   if (op->q2)
@@ -466,6 +531,12 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+  if (op->n_constrained)
+    {
+      hps::constraint_restore32(op->stream, op->n_constrained, op->nc, op->constrained,
+                                const_cast<float *>(src), dst, save32);
+      HPS_LAUNCHED();
+    }
   return HPS_OK;
 }
 
@@ -505,6 +576,9 @@ static int precond_update(hps_operator op, int kind, bool fp32)
         }
       else if (int r = finish_fixup<double>(op, op->diag, nc, 1, 0))
         return r;
+      // identity rows of the constrained nodes: diagonal 1
+      if (op->n_constrained)
+        hps::constraint_set(op->stream, op->n_constrained, nc, op->constrained, op->diag, 1.0);
       hps::invert_diag(op->stream, n, op->diag, op->inv_diag64, op->inv_diag32);
       HPS_LAUNCHED();
     }
@@ -538,6 +612,10 @@ static int precond_update(hps_operator op, int kind, bool fp32)
         }
       if (fp32)
         hps::blas_f32_to_f64(op->stream, nn * nc * nc, op->blocks, op->inv_blocks32);
+      // identity rows and columns of the constrained nodes: identity blocks
+      if (op->n_constrained)
+        hps::constraint_identity_blocks(op->stream, op->n_constrained, nc, op->constrained,
+                                        op->blocks);
       HPS_HIP(hipMemsetAsync(op->d_singular, 0, sizeof(int), op->stream));
       if (!hps::invert_blocks(op->stream, nn, nc, op->blocks, op->inv_blocks64,
                               op->inv_blocks32, op->d_singular))

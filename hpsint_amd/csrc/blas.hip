@@ -687,4 +687,101 @@ namespace hps
     HPS_K_SWITCH(k, HPS_CALL)
 #undef HPS_CALL
   }
+
+  // ------------------------------------------------ constrained nodes
+  // OperatorBase::vmult (S/operator_base.h:294-353): before the cell
+  // operator the constrained entries of src are saved and zeroed; after it
+  // src is restored and dst equals src on them (identity rows).  One item
+  // per (constrained node, component).
+  template <typename T>
+  __global__ void k_constraint_zero(int64_t n, int nc, const int64_t *nodes, T *src, T *save)
+  {
+    const int64_t total = n * nc;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+      {
+        const int64_t k = i / nc;
+        T *p            = src + nodes[k] * nc + (i - k * nc);
+        save[i]         = *p;
+        *p              = T(0);
+      }
+  }
+
+  template <typename T>
+  __global__ void k_constraint_restore(int64_t n, int nc, const int64_t *nodes, T *src, T *dst,
+                                       const T *save)
+  {
+    const int64_t total = n * nc;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+      {
+        const int64_t k   = i / nc;
+        const int64_t idx = nodes[k] * nc + (i - k * nc);
+        src[idx]          = save[i];
+        if (dst)
+          dst[idx] = save[i];
+      }
+  }
+
+  // constrained rows of dst set to value (residual rows: 0; point diagonal: 1)
+  __global__ void k_constraint_set(int64_t n, int nc, const int64_t *nodes, double *dst,
+                                   double value)
+  {
+    const int64_t total = n * nc;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+      {
+        const int64_t k       = i / nc;
+        dst[nodes[k] * nc + (i - k * nc)] = value;
+      }
+  }
+
+  // node blocks of constrained nodes set to the identity
+  __global__ void k_constraint_identity_blocks(int64_t n, int nc, const int64_t *nodes,
+                                               double *blocks)
+  {
+    const int64_t per   = (int64_t)nc * nc;
+    const int64_t total = n * per;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+      {
+        const int64_t k = i / per;
+        const int     e = (int)(i - k * per);
+        blocks[nodes[k] * per + e] = (e / nc == e % nc) ? 1.0 : 0.0;
+      }
+  }
+
+  void constraint_zero64(hipStream_t s, int64_t n, int nc, const int64_t *nodes, double *src,
+                         double *save)
+  {
+    k_constraint_zero<double><<<grid_for(n * nc, 256), 256, 0, s>>>(n, nc, nodes, src, save);
+  }
+  void constraint_zero32(hipStream_t s, int64_t n, int nc, const int64_t *nodes, float *src,
+                         float *save)
+  {
+    k_constraint_zero<float><<<grid_for(n * nc, 256), 256, 0, s>>>(n, nc, nodes, src, save);
+  }
+  void constraint_restore64(hipStream_t s, int64_t n, int nc, const int64_t *nodes, double *src,
+                            double *dst, const double *save)
+  {
+    k_constraint_restore<double>
+      <<<grid_for(n * nc, 256), 256, 0, s>>>(n, nc, nodes, src, dst, save);
+  }
+  void constraint_restore32(hipStream_t s, int64_t n, int nc, const int64_t *nodes, float *src,
+                            float *dst, const float *save)
+  {
+    k_constraint_restore<float>
+      <<<grid_for(n * nc, 256), 256, 0, s>>>(n, nc, nodes, src, dst, save);
+  }
+  void constraint_set(hipStream_t s, int64_t n, int nc, const int64_t *nodes, double *dst,
+                      double value)
+  {
+    k_constraint_set<<<grid_for(n * nc, 256), 256, 0, s>>>(n, nc, nodes, dst, value);
+  }
+  void constraint_identity_blocks(hipStream_t s, int64_t n, int nc, const int64_t *nodes,
+                                  double *blocks)
+  {
+    k_constraint_identity_blocks<<<grid_for(n * nc * nc, 256), 256, 0, s>>>(n, nc, nodes,
+                                                                             blocks);
+  }
 } // namespace hps

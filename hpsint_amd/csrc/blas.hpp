@@ -46,4 +46,17 @@ namespace hps
                            double *y);
   void blas_axpy_multi_f32_f64(hipStream_t, int64_t n, int k, const double *a, const float *X,
                                double *y);
+  // constrained nodes (node indices, each listed once)
+  void constraint_zero64(hipStream_t, int64_t n, int nc, const int64_t *nodes, double *src,
+                         double *save);
+  void constraint_zero32(hipStream_t, int64_t n, int nc, const int64_t *nodes, float *src,
+                         float *save);
+  void constraint_restore64(hipStream_t, int64_t n, int nc, const int64_t *nodes, double *src,
+                            double *dst, const double *save);
+  void constraint_restore32(hipStream_t, int64_t n, int nc, const int64_t *nodes, float *src,
+                            float *dst, const float *save);
+  void constraint_set(hipStream_t, int64_t n, int nc, const int64_t *nodes, double *dst,
+                      double value);
+  void constraint_identity_blocks(hipStream_t, int64_t n, int nc, const int64_t *nodes,
+                                  double *blocks);
 } // namespace hps

@@ -13,6 +13,7 @@ arithmetic runs in the HIP kernels of libhpsint_amd.so.
 import ctypes
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 
 from . import _ffi
@@ -233,6 +234,15 @@ class SinteringOperatorGeneric:
         old = self._old if old is None else old
         arr = (ctypes.c_void_p * max(1, len(old)))(*[o.data_ptr() for o in old])
         check(lib().hps_residual(self._h, with_time_derivative, _ptr(dst), _ptr(src), arr))
+
+    def set_constrained_nodes(self, nodes):
+        """constrained nodes (OperatorBase::vmult's constrained_indices from
+        MatrixFree::get_constrained_dofs, S/operator_base.h:300-337): vmult
+        treats their src entries as zero and returns dst = src on them,
+        the residual leaves their rows zero; an empty list removes them"""
+        arr = np.ascontiguousarray(np.asarray(nodes, dtype=np.int64).ravel())
+        check(lib().hps_set_constrained_nodes(
+            self._h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(arr.size)))
 
     def vmult(self, dst, src):
         if src.dtype == torch.float32:

@@ -109,6 +109,17 @@ int hps_set_linearization(hps_operator op, const double *u);
 int hps_residual(hps_operator op, int wtd, double *dst, const double *src,
                  const double *const *old);
 
+/* Constrained nodes: OperatorBase::vmult's constrained_indices, from
+ * MatrixFree::get_constrained_dofs (S/operator_base.h:300-314).  nodes is a
+ * host array of n distinct node indices (n = 0: none, the default).  With
+ * constraints, hps_vmult_* zero the constrained entries of src for the cell
+ * operator (src is modified during the call and restored before the call's
+ * work on the stream ends) and set dst = src on them (identity rows,
+ * S/operator_base.h:316-337); hps_residual leaves their rows zero
+ * (FEEvaluation::distribute_local_to_global skips constrained entries); the
+ * preconditioners use diagonal 1 and identity node blocks for them. */
+int hps_set_constrained_nodes(hps_operator op, const int64_t *nodes, int64_t n);
+
 /* OperatorBase::vmult -> SinteringOperatorGeneric::vmult_internal
  * (S/operator_base.h:294-353, S/operator_sintering_generic.h:579-606):
  * tangent at the linearisation point; dst is overwritten.  The f32 variant

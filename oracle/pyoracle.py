@@ -198,6 +198,28 @@ def vmult(p, cache, src):
     return dst
 
 
+def vmult_constrained(p, cache, src, nodes):
+    """OperatorBase::vmult with constrained DoFs (S/operator_base.h:294-353):
+    the constrained entries of src (all components of the listed nodes) are
+    zero for the cell operator, and dst equals src on them afterwards."""
+    nc = p.n_comp
+    s = np.array(src, dtype=np.float64).reshape(-1, nc)
+    saved = s[nodes].copy()
+    s[nodes] = 0.0
+    dst = vmult(p, cache, s.ravel()).reshape(-1, nc)
+    dst[nodes] = saved
+    return dst.ravel()
+
+
+def residual_constrained(p, wtd, src, old, nodes):
+    """residual with constrained DoFs: FEEvaluation::distribute_local_to_global
+    (deal.II, called at S/operator_sintering_generic.h:1005) skips constrained
+    entries, so their rows stay zero"""
+    dst = residual(p, wtd, src, old).reshape(-1, p.n_comp)
+    dst[nodes] = 0.0
+    return dst.ravel()
+
+
 def node_blocks(p, cache):
     nn = n_nodes(p)
     out = np.empty(nn * p.n_comp * p.n_comp)

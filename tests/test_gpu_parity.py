@@ -356,3 +356,81 @@ def test_fp32_preconditioner_assembly(dim, cells, G, tens):
                       1.0 / np.einsum("nii->ni", blocks_ref).ravel(), 1.0)
     _ffi.check(_ffi.lib().hps_precond_get_diagonal(op.handle, ctypes.c_void_p(inv.data_ptr())))
     assert rel_l2(host(inv), op_inv) <= 1e-5
+
+
+def boundary_and_random_nodes(p, seed, k=40):
+    """the nodes of the first mesh face plus k random nodes, distinct"""
+    nn = po.n_nodes(p)
+    n0 = p.n_cells[0] + 1 if p.degree == 1 else 2 * p.n_cells[0] + 1
+    face = np.arange(0, nn, n0)                      # x = 0 face
+    rnd = np.random.default_rng(seed).choice(nn, size=min(k, nn), replace=False)
+    return np.unique(np.concatenate([face, rnd]))
+
+
+@pytest.mark.parametrize("degree,dim,cells,G,tens", [
+    (1, 3, (9, 5, 17), 2, False), (1, 3, (11, 6, 18), 3, True), (1, 2, (65, 17), 2, False),
+    (2, 3, (5, 4, 7), 2, False), (2, 3, (4, 3, 5), 3, True)])
+def test_constrained_nodes_vmult_residual_preconditioner(degree, dim, cells, G, tens):
+    """S/operator_base.h:294-353: zeroed constrained src entries, identity rows,
+    src restored bitwise; residual rows of constrained nodes zero; diagonal 1
+    and identity node blocks on them"""
+    if degree == 1:
+        op, p = make_pair(dim, cells, G, tens, w0=100.0, w_hist=(-100.0,))
+    else:
+        h = (0.5, 0.75, 1.0)[:dim]
+        fe, mob = FreeEnergyParams(), MobilityParams(tensorial=tens)
+        mesh = CartesianMesh(tuple(cells), (0.0,) * dim,
+                             tuple(n * hh for n, hh in zip(cells, h)))
+        op = SinteringOperatorGeneric(mesh, G, fe, mob, fe_degree=2)
+        op.set_time_weights(100.0, (-100.0,))
+        p = po.make_params(dim, 2, cells, h, G + 2, tensorial=tens, A=fe.A, B=fe.B,
+                           kappa_c=fe.kappa_c, kappa_p=fe.kappa_p, Mvol=mob.Mvol,
+                           Mvap=mob.Mvap, Msurf=mob.Msurf, Mgb=mob.Mgb, L=mob.L, w0=100.0,
+                           w_hist=(-100.0,))
+    nc = p.n_comp
+    nodes = boundary_and_random_nodes(p, seed=5)
+    u = fields(p, 1)
+    src = np.random.default_rng(2).uniform(-1, 1, size=u.size)
+    cache = po.fill_qp(p, u)
+    op.fill_quadrature_point_values(dev(u))
+    op.set_constrained_nodes(nodes)
+    s = dev(src)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, s)
+    ref = po.vmult_constrained(p, cache, src, nodes)
+    got = host(dst)
+    assert rel_l2(got, ref) <= TOL64, rel_l2(got, ref)
+    assert np.array_equal(host(s), src)                      # src restored
+    assert np.array_equal(got.reshape(-1, nc)[nodes], src.reshape(-1, nc)[nodes])
+    # FP32
+    s32 = dev(src, torch.float32)
+    d32 = op.initialize_dof_vector(torch.float32)
+    op.vmult(d32, s32)
+    assert np.array_equal(host(s32), src.astype(np.float32).astype(np.float64))
+    g32 = host(d32).reshape(-1, nc)
+    r2 = ref.reshape(-1, nc)
+    assert max(rel_l2(g32[:, c], r2[:, c]) for c in range(nc)) <= 1e-5
+    # residual
+    old = fields(p, 3)
+    r = op.initialize_dof_vector()
+    op.evaluate_nonlinear_residual(r, dev(u), 2, old=[dev(old)])
+    rr = po.residual_constrained(p, 2, u, [old], nodes)
+    assert rel_l2(host(r), rr) <= TOL64
+    # preconditioners: identity on the constrained nodes
+    inv = op.initialize_dof_vector()
+    op.compute_inverse_diagonal(inv)
+    assert np.all(host(inv).reshape(-1, nc)[nodes] == 1.0)
+    B = host(op.node_blocks()).reshape(-1, nc, nc)
+    assert np.array_equal(B[nodes], np.broadcast_to(np.eye(nc), (nodes.size, nc, nc)))
+    # removing the constraints restores the plain operator
+    op.set_constrained_nodes([])
+    op.vmult(dst, s)
+    assert rel_l2(host(dst), po.vmult(p, cache, src)) <= TOL64
+
+
+def test_constrained_nodes_reject_bad_lists():
+    op, p = make_pair(3, (3, 3, 3), 2)
+    nn = po.n_nodes(p)
+    for bad in ([0, 0], [-1], [nn]):
+        with pytest.raises(RuntimeError):
+            op.set_constrained_nodes(bad)

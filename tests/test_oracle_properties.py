@@ -224,3 +224,33 @@ def test_vmult_is_linear_and_history_cancels():
     r2 = po.residual(p, 2, u, [u])
     r0 = po.residual(p, 0, u)
     np.testing.assert_allclose(r2, r0, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,degree,cells", [(3, 1, (2, 2, 2)), (2, 2, (2, 1))])
+def test_constrained_vmult_is_the_matrix_with_identity_rows(dim, degree, cells):
+    """S/operator_base.h:294-353 as a matrix: the constrained columns of the
+    tangent are dropped and the constrained rows become unit rows; all nodes
+    constrained gives the identity, none gives the plain tangent."""
+    p = po.make_params(dim, degree, cells, (0.7, 1.0, 1.3)[:dim], 5, w0=1e3,
+                       w_hist=(-1e3,))
+    u = rand_state(p, seed=8)
+    cache = po.fill_qp(p, u)
+    J = po.dense_tangent(p, cache)
+    nc, nn = p.n_comp, po.n_nodes(p)
+    src = np.random.default_rng(9).standard_normal(nn * nc)
+    nodes = np.array([0, 3, nn - 1])
+    dofs = (nodes[:, None] * nc + np.arange(nc)).ravel()
+    Jc = J.copy()
+    Jc[:, dofs] = 0.0
+    Jc[dofs, :] = 0.0
+    Jc[dofs, dofs] = 1.0
+    np.testing.assert_allclose(po.vmult_constrained(p, cache, src, nodes), Jc @ src,
+                               rtol=1e-12, atol=1e-12 * np.abs(J).max())
+    np.testing.assert_array_equal(po.vmult_constrained(p, cache, src, np.arange(nn)), src)
+    np.testing.assert_array_equal(po.vmult_constrained(p, cache, src, np.array([], int)),
+                                  po.vmult(p, cache, src))
+    r = po.residual_constrained(p, 2, u, [u], nodes).reshape(nn, nc)
+    assert not r[nodes].any()
+    np.testing.assert_array_equal(np.delete(r, nodes, axis=0),
+                                  np.delete(po.residual(p, 2, u, [u]).reshape(nn, nc),
+                                            nodes, axis=0))
