@@ -228,7 +228,8 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
             qp.h_prod *= prm->cell_size[d];
         }
       op->cache_elems =
-        hps::q2_workgroups(qp, prm->dim, q2t->cpw) * (int64_t)q2t->n_fields * 64;
+        hps::q2_workgroups(qp, prm->dim, q2t->cpw) * (int64_t)q2t->n_fields *
+        (q2t->cpw * (prm->dim == 3 ? 27 : 9));
       op->part_elems = 0;
     }
   else

@@ -149,6 +149,7 @@ namespace hps
   {
     static constexpr int NP  = DIM == 3 ? 27 : 9;
     static constexpr int CPW = 64 / NP; // cells per wave
+    static constexpr int LP  = CPW * NP; // lanes that hold a cell's node / point
     static constexpr int str(int d)
     {
       return d == 0 ? 1 : (d == 1 ? 3 : 9);
@@ -307,159 +308,200 @@ namespace hps
     return r;
   }
 
-  // the workgroup's cells (the same for every lane): node index of local
-  // node 0 of each cell slot (-1: no cell) and the cell coordinates, so that
-  // the per-item index work of a gather or scatter is a few multiply-adds
-  template <int DIM>
-  struct Q2Cells
+  // 16-byte vector of T
+  template <typename T>
+  struct Vec16;
+  template <>
+  struct Vec16<double>
   {
-    using LN = Q2Lane<DIM>;
-    int64_t n0[LN::CPW];
-    int     cc[LN::CPW][3];
-    int64_t sy, sz;
-
-    __device__ explicit Q2Cells(const Q2Params &kp)
-    {
-      sy = kp.nn[0];
-      sz = (int64_t)kp.nn[0] * kp.nn[1];
-#pragma unroll
-      for (int k = 0; k < LN::CPW; ++k)
-        {
-          int c[3] = {0, 0, 0};
-          n0[k]    = LN::cell(kp, (int64_t)blockIdx.x * LN::CPW + k, c) ? LN::node(kp, c, 0) : -1;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            cc[k][d] = c[d];
-        }
-    }
-    // node of local node p of cell slot s (-1: no cell); c: the cell
-    __device__ int64_t node(int s, int p, int *c) const
-    {
-      int64_t b = -1;
-      c[0] = c[1] = c[2] = 0;
-#pragma unroll
-      for (int k = 0; k < LN::CPW; ++k)
-        if (s == k)
-          {
-            b    = n0[k];
-            c[0] = cc[k][0];
-            c[1] = cc[k][1];
-            c[2] = cc[k][2];
-          }
-      const int tx = p % 3, ty = (p / 3) % 3, tz = DIM == 3 ? p / 9 : 0;
-      return b < 0 ? int64_t(-1) : b + tx + ty * sy + tz * sz;
-    }
+    using type = double2;
+  };
+  template <>
+  struct Vec16<float>
+  {
+    using type = float4;
   };
 
-  // gather n_comp components of the cells of this workgroup into
-  // buf[c * 64 + lane] (lane order); missing cells read as zero
+  // the NC consecutive values at p (16-byte loads where the row allows them)
+  template <int NC, typename T>
+  __device__ __forceinline__ void q2_load_row(const T *p, T *x)
+  {
+    constexpr int W = 16 / (int)sizeof(T);
+    if constexpr (NC % W == 0)
+      if ((reinterpret_cast<uintptr_t>(p) & 15) == 0)
+        {
+#pragma unroll
+          for (int k = 0; k < NC / W; ++k)
+            {
+              const typename Vec16<T>::type v =
+                reinterpret_cast<const typename Vec16<T>::type *>(p)[k];
+              __builtin_memcpy(x + k * W, &v, 16);
+            }
+          return;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      x[c] = p[c];
+  }
+
+  template <int NC, typename T>
+  __device__ __forceinline__ void q2_store_row(T *p, const T *x)
+  {
+    constexpr int W = 16 / (int)sizeof(T);
+    if constexpr (NC % W == 0)
+      if ((reinterpret_cast<uintptr_t>(p) & 15) == 0)
+        {
+#pragma unroll
+          for (int k = 0; k < NC / W; ++k)
+            {
+              typename Vec16<T>::type v;
+              __builtin_memcpy(&v, x + k * W, 16);
+              reinterpret_cast<typename Vec16<T>::type *>(p)[k] = v;
+            }
+          return;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      p[c] = x[c];
+  }
+
+  // gather: each lane of a cell slot reads the NC components of its own
+  // node (row(node, x) fills x[NC]) into buf[c * 64 + lane]; the other
+  // lanes write zeros.  Two cells of one colour share no node.
   template <int DIM, int NC, typename T, typename F>
-  __device__ __forceinline__ void q2_gather(const Q2Params &kp, T *buf, F &&value)
+  __device__ __forceinline__ void q2_gather(const Q2Params &kp, const Q2Lane<DIM> &L, T *buf,
+                                            F &&row)
   {
-    using LN         = Q2Lane<DIM>;
-    constexpr int NI = LN::CPW * LN::NP * NC;
-    const Q2Cells<DIM> wc(kp);
+    T x[NC];
+    if (L.valid)
+      row(Q2Lane<DIM>::node(kp, L.cc, L.p), x);
+    else
 #pragma unroll
-    for (int it = 0; it < (NI + 63) / 64; ++it)
-      {
-        const int f = it * 64 + (int)threadIdx.x;
-        if (f < NI)
-          {
-            const int     sp = f / NC, c = f - sp * NC;
-            const int     s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
-            int           cc[3];
-            const int64_t n = wc.node(s2, p2, cc);
-            T             v = T(0);
-            if (n >= 0)
-              v = value(n * NC + c);
-            buf[c * 64 + sp] = v;
-          }
-      }
+      for (int c = 0; c < NC; ++c)
+        x[c] = T(0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      buf[c * 64 + L.l] = x[c];
   }
 
-  // store (first colour) or add buf[c * 64 + lane] to
-  // dst[node * os + c * ocs + oo]
+  // scatter: each lane of an existing cell stores (first colour to reach the
+  // node) or adds buf[c * 64 + lane] to dst[node * os + c * ocs + oo]
   template <int DIM, int NC, typename T>
-  __device__ __forceinline__ void q2_scatter(const Q2Params &kp, const T *buf, T *dst, int os,
-                                             int ocs, int oo)
+  __device__ __forceinline__ void q2_scatter(const Q2Params &kp, const Q2Lane<DIM> &L,
+                                             const T *buf, T *dst, int os, int ocs, int oo)
   {
-    using LN         = Q2Lane<DIM>;
-    constexpr int NI = LN::CPW * LN::NP * NC;
-    const Q2Cells<DIM> wc(kp);
+    using LN = Q2Lane<DIM>;
+    if (!L.valid)
+      return;
+    const int64_t n     = LN::node(kp, L.cc, L.p);
+    const bool    first = LN::first(kp, L.cc, L.p);
+    T             r[NC];
 #pragma unroll
-    for (int it = 0; it < (NI + 63) / 64; ++it)
+    for (int c = 0; c < NC; ++c)
+      r[c] = buf[c * 64 + L.l];
+    if (os == NC && ocs == 1 && oo == 0)
       {
-        const int f = it * 64 + (int)threadIdx.x;
-        if (f < NI)
+        T *d = dst + n * NC;
+        if (!first)
           {
-            const int     sp = f / NC, c = f - sp * NC;
-            const int     s2 = sp / LN::NP, p2 = sp - s2 * LN::NP;
-            int           cc[3];
-            const int64_t n = wc.node(s2, p2, cc);
-            if (n >= 0)
-              {
-                T      *d = dst + n * os + (int64_t)c * ocs + oo;
-                const T r = buf[c * 64 + sp];
-                *d        = LN::first(kp, cc, p2) ? r : *d + r;
-              }
+            T o[NC];
+            q2_load_row<NC>(d, o);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              r[c] = o[c] + r[c];
           }
+        q2_store_row<NC>(d, r);
       }
+    else
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        {
+          T *d = dst + n * os + (int64_t)c * ocs + oo;
+          *d   = first ? r[c] : *d + r[c];
+        }
   }
 
-  // tangent vmult of the workgroup's cells: U holds the gathered increment
-  // on entry and the integrated rows on exit; VR (NC rows of 64) takes the
-  // value rows.  Only the gradients of c and mu are interpolated with the
-  // values; the Gauss-point values of eta_i replace their nodal values in U
-  // and the gradient of eta_i is formed from them right before row i is
-  // integrated (it enters that row only, through L kappa_p grad eta_i).  The
-  // cache is read once, for the value rows and the c gradient row.  Few
-  // values stay live per lane.
+  // tangent vmult of the workgroup's cells.  U holds the gathered increment
+  // (NC rows of 64) on entry and the integrated rows on exit; A (NC rows) is
+  // the second work buffer; VR (NC rows) takes the value rows.  Every
+  // sum-factorisation pass runs over all components between two barriers,
+  // so a workgroup waits for LDS once per pass instead of once per
+  // component and pass.  Only the gradients of c and mu enter the cached
+  // tangent; the gradient of eta_i enters its own row only (L kappa_p
+  // grad eta_i) and is formed from the Gauss-point values before the
+  // integration.
   template <int DIM, int G, bool TENS, typename T>
   __device__ __forceinline__ void q2_vmult_body(const Q2Params &kp, const Q2Lane<DIM> &L, T *U,
-                                                T *VR, T (*W)[64], const T *__restrict__ cache,
+                                                T *VR, T *A, const T *__restrict__ cache,
                                                 const T *inv_h)
   {
     using CL         = CacheLayout<DIM, G, TENS>;
+    using LN         = Q2Lane<DIM>;
     constexpr int NC = G + 2;
     constexpr int NF = CL::n_fields;
     const int     l  = L.l;
-    T             v0, v1, g0[DIM], g1[DIM];
+
+    // 1) values at the Gauss points, one pass per direction for all
+    //    components (U -> A -> U -> registers in 3D, U -> A -> registers in 2D)
+    T v[NC];
     {
-      T srow[DIM][3], drow[DIM][3];
+      T srow[DIM][3];
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
-        {
-          q2_s_row(L.t[d], srow[d]);
-          q2_dq_row(L.t[d], drow[d]);
-        }
-      q2_interp_vg<DIM>(L, U, W[0], W[1], W[2], srow, drow, inv_h, v0, g0);
-      q2_interp_vg<DIM>(L, U + 64, W[0], W[1], W[2], srow, drow, inv_h, v1, g1);
-      // (q2_interp_v ends after a barrier that follows every read of the
-      // nodal row, so the row can take the Gauss-point values)
+        q2_s_row(L.t[d], srow[d]);
 #pragma unroll
-      for (int c = 2; c < NC; ++c)
+      for (int c = 0; c < NC; ++c)
+        A[c * 64 + l] = q2_pass(U + c * 64, L.base(0), 1, srow[0]);
+      __syncthreads();
+      if constexpr (DIM == 3)
         {
-          const T x    = q2_interp_v<DIM>(L, U + c * 64, W[0], W[1], srow);
-          U[c * 64 + l] = x;
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            U[c * 64 + l] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
+          __syncthreads();
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            v[c] = q2_pass(U + c * 64, L.base(2), 9, srow[2]);
         }
+      else
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          v[c] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
     }
+    // QV: the buffer the last pass did not read takes the Gauss-point
+    // values; WS is the other one
+    T *QV = DIM == 3 ? A : U;
+    T *WS = DIM == 3 ? U : A;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      QV[c * 64 + l] = v[c];
+    __syncthreads();
+
+    T drow[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      q2_dq_row(L.t[d], drow[d]);
+    T g0[DIM], g1[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        g0[d] = q2_pass(QV, L.base(d), LN::str(d), drow[d]) * inv_h[d];
+        g1[d] = q2_pass(QV + 64, L.base(d), LN::str(d), drow[d]) * inv_h[d];
+      }
+
+    // 2) cache and tangent: value rows (times JxW) to VR, the c gradient row
     T jxw = T(kp.h_prod);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       jxw *= q2_w<T>(L.t[d]);
-    T        gc0[DIM];
-    const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + l;
+    T             gq[NC][DIM];
+    constexpr int LP = LN::LP; // cache: [workgroup][field][LP lanes]
+    const T      *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LP + l;
     {
-      T v[NC], vr[NC];
-      v[0] = v0;
-      v[1] = v1;
-#pragma unroll
-      for (int c = 2; c < NC; ++c)
-        v[c] = U[c * 64 + l];
+      T vr[NC], gc0[DIM];
       if constexpr (TENS && G > 1)
         {
-          // sum_i deta_i Q_i streamed from the cache first; the barrier
-          // keeps the loads of the other fields below it
+          // sum_i deta_i Q_i streamed from the cache first
           T qsum[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; ++d)
@@ -468,31 +510,46 @@ namespace hps
           for (int i = 0; i < G; ++i)
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-              qsum[d] += v[2 + i] * (L.valid ? cl[(CL::gb_q + i * DIM + d) * 64] : T(0));
-          __syncthreads();
+              qsum[d] += v[2 + i] * (L.valid ? cl[(CL::gb_q + i * DIM + d) * LP] : T(0));
           T lin[CL::gb_q];
 #pragma unroll
           for (int f = 0; f < CL::gb_q; ++f)
-            lin[f] = L.valid ? cl[f * 64] : T(0);
+            lin[f] = L.valid ? cl[f * LP] : T(0);
           tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
           tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0, qsum);
-#pragma unroll
-          for (int c = 0; c < NC; ++c)
-            VR[c * 64 + l] = vr[c] * jxw;
         }
       else
         {
           T lin[NF];
 #pragma unroll
           for (int f = 0; f < NF; ++f)
-            lin[f] = L.valid ? cl[f * 64] : T(0);
+            lin[f] = L.valid ? cl[f * LP] : T(0);
           tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
           tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0);
+        }
 #pragma unroll
-          for (int c = 0; c < NC; ++c)
-            VR[c * 64 + l] = vr[c] * jxw;
+      for (int c = 0; c < NC; ++c)
+        VR[c * 64 + l] = vr[c] * jxw;
+      // 3) gradient rows with JxW / h_d: c, mu = -kappa_c grad c,
+      //    eta_i = L kappa_p grad eta_i (from the Gauss-point values in QV)
+      const T kc  = -T(kp.pp.kappa_c);
+      const T lkp = T(kp.pp.L) * T(kp.pp.kappa_p);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        {
+          const T sd = jxw * inv_h[d];
+          gq[0][d]   = gc0[d] * sd;
+          gq[1][d]   = kc * g0[d] * sd;
+#pragma unroll
+          for (int c = 2; c < NC; ++c)
+            gq[c][d] =
+              q2_pass(QV + c * 64, L.base(d), LN::str(d), drow[d]) * (lkp * inv_h[d] * sd);
         }
     }
+    __syncthreads();
+
+    // 4) integration: collocation-derivative transposes of the gradient rows
+    //    added to the value rows, then the transposed interpolation passes
     T scol[DIM][3], dcol[DIM][3];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
@@ -500,33 +557,68 @@ namespace hps
         q2_s_col(L.t[d], scol[d]);
         q2_dq_col(L.t[d], dcol[d]);
       }
-    T gq[DIM];
+    T y[NC];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      gq[d] = gc0[d] * (jxw * inv_h[d]);
-    U[l] = q2_integrate<DIM>(L, VR[l], gq, W[0], W[1], W[2], W[3], scol, dcol);
-    const T kc = -T(kp.pp.kappa_c);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      gq[d] = kc * g0[d] * (jxw * inv_h[d]);
-    U[64 + l] = q2_integrate<DIM>(L, VR[64 + l], gq, W[0], W[1], W[2], W[3], scol, dcol);
-    const T lkp = T(kp.pp.L) * T(kp.pp.kappa_p) * jxw;
-#pragma unroll 1
-    for (int c = 2; c < NC; ++c)
+    for (int c = 0; c < NC; ++c)
       {
-        // gradient of eta_c from its Gauss-point values in row c (written
-        // before earlier barriers; read before the first barrier of
-        // q2_integrate, whose result replaces the row after its last one)
+        WS[c * 64 + l] = gq[c][0];
+        QV[c * 64 + l] = gq[c][1];
+      }
+    __syncthreads();
 #pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          {
-            T dr[3];
-            q2_dq_row(L.t[d], dr);
-            gq[d] = q2_pass(U + c * 64, L.base(d), Q2Lane<DIM>::str(d), dr) *
-                    (lkp * inv_h[d] * inv_h[d]);
-          }
-        U[c * 64 + l] =
-          q2_integrate<DIM>(L, VR[c * 64 + l], gq, W[0], W[1], W[2], W[3], scol, dcol);
+    for (int c = 0; c < NC; ++c)
+      y[c] = VR[c * 64 + l] + q2_pass(WS + c * 64, L.base(0), 1, dcol[0]) +
+             q2_pass(QV + c * 64, L.base(1), 3, dcol[1]);
+    __syncthreads();
+    if constexpr (DIM == 3)
+      {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          WS[c * 64 + l] = gq[c][2];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] += q2_pass(WS + c * 64, L.base(2), 9, dcol[2]);
+        __syncthreads();
+        // z, y, x: QV -> WS -> QV -> U (= WS)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          QV[c * 64 + l] = y[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          WS[c * 64 + l] = q2_pass(QV + c * 64, L.base(2), 9, scol[2]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          QV[c * 64 + l] = q2_pass(WS + c * 64, L.base(1), 3, scol[1]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] = q2_pass(QV + c * 64, L.base(0), 1, scol[0]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          U[c * 64 + l] = y[c];
+      }
+    else
+      {
+        // y, x: QV -> WS -> U (= QV)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          QV[c * 64 + l] = y[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          WS[c * 64 + l] = q2_pass(QV + c * 64, L.base(1), 3, scol[1]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] = q2_pass(WS + c * 64, L.base(0), 1, scol[0]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          U[c * 64 + l] = y[c];
       }
   }
 
@@ -544,27 +636,42 @@ namespace hps
 
     __shared__ T U[NC * 64];
     __shared__ T H[(HIST || OP == OP_VMULT) ? NC * 64 : 1];
-    __shared__ T W[4][64];
+    __shared__ T W[OP == OP_VMULT ? 1 : 4][64];
+    __shared__ T A[OP == OP_VMULT ? NC * 64 : 1];
 
     const Q2Lane<DIM> L(kp);
-    q2_gather<DIM, NC, T>(kp, U, [&](int64_t i) { return src[i]; });
+    q2_gather<DIM, NC, T>(kp, L, U, [&](int64_t n, T *x) { q2_load_row<NC>(src + n * NC, x); });
     if constexpr (HIST)
-      q2_gather<DIM, NC, T>(kp, H, [&](int64_t i) {
-        T h = old0[i] * T(kp.w_hist[0]);
+      q2_gather<DIM, NC, T>(kp, L, H, [&](int64_t n, T *x) {
+        // do_evalute_history, S/operator_sintering_generic.h:950-960
+        T o[NC];
+        q2_load_row<NC>(old0 + n * NC, o);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          x[c] = o[c] * T(kp.w_hist[0]);
         if (kp.order > 1)
-          h += old1[i] * T(kp.w_hist[1]);
+          {
+            q2_load_row<NC>(old1 + n * NC, o);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              x[c] += o[c] * T(kp.w_hist[1]);
+          }
         if (kp.order > 2)
-          h += old2[i] * T(kp.w_hist[2]);
-        return h;
+          {
+            q2_load_row<NC>(old2 + n * NC, o);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              x[c] += o[c] * T(kp.w_hist[2]);
+          }
       });
     __syncthreads();
 
     const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
     if constexpr (OP == OP_VMULT)
       {
-        q2_vmult_body<DIM, G, TENS>(kp, L, U, H, W, cache, inv_h);
+        q2_vmult_body<DIM, G, TENS>(kp, L, U, H, A, cache, inv_h);
         __syncthreads();
-        q2_scatter<DIM, NC, T>(kp, U, dst, NC, 1, 0);
+        q2_scatter<DIM, NC, T>(kp, L, U, dst, NC, 1, 0);
         return;
       }
     T v[NC], g[NC][DIM];
@@ -589,10 +696,10 @@ namespace hps
     T lin[OP == OP_VMULT ? NF : 1];
     if constexpr (OP == OP_VMULT)
       {
-        const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
+        const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * Q2Lane<DIM>::LP + L.l;
 #pragma unroll
         for (int f = 0; f < NF; ++f)
-          lin[f] = L.valid ? cl[f * 64] : T(0);
+          lin[f] = L.valid ? cl[f * Q2Lane<DIM>::LP] : T(0);
         tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
       }
     else
@@ -656,7 +763,7 @@ namespace hps
         }
     }
     __syncthreads();
-    q2_scatter<DIM, NC, T>(kp, U, dst, NC, 1, 0);
+    q2_scatter<DIM, NC, T>(kp, L, U, dst, NC, 1, 0);
   }
 
   // ------------------------------------------------ linearisation cache
@@ -674,7 +781,7 @@ namespace hps
     __shared__ T W[3][64];
 
     const Q2Lane<DIM> L(kp);
-    q2_gather<DIM, NC, T>(kp, U, [&](int64_t i) { return lin[i]; });
+    q2_gather<DIM, NC, T>(kp, L, U, [&](int64_t n, T *x) { q2_load_row<NC>(lin + n * NC, x); });
     __syncthreads();
 
     const T inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
@@ -718,13 +825,19 @@ namespace hps
     if constexpr (TENS)
       gb_fields<DIM, G>(f + CL::eta, leg, f + CL::grad_mu, f[CL::gb_s], f + CL::gb_p,
                         f + CL::gb_q);
-    const int64_t o = ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
-#pragma unroll
-    for (int q = 0; q < NF; ++q)
+    // cache: [workgroup][field][LP lanes]; lanes past the cell slots hold
+    // nothing
+    constexpr int LP = Q2Lane<DIM>::LP;
+    if (L.l < LP)
       {
-        cache64[o + q * 64] = f[q];
-        if (cache32)
-          cache32[o + q * 64] = (float)f[q];
+        const int64_t o = ((kp.wg_base + blockIdx.x) * NF) * LP + L.l;
+#pragma unroll
+        for (int q = 0; q < NF; ++q)
+          {
+            cache64[o + q * LP] = f[q];
+            if (cache32)
+              cache32[o + q * LP] = (float)f[q];
+          }
       }
   }
 
@@ -756,10 +869,10 @@ namespace hps
     const LN L(kp);
     T        lin[NF];
     {
-      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * 64 + L.l;
+      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LN::LP + L.l;
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        lin[f] = L.valid ? cl[f * 64] : T(0);
+        lin[f] = L.valid ? cl[f * LN::LP] : T(0);
     }
     T jxw = T(kp.h_prod);
 #pragma unroll
@@ -874,6 +987,6 @@ namespace hps
     for (int a = 0; a < NC; ++a)
       R[a * 64 + L.l] = acc[a];
     __syncthreads();
-    q2_scatter<DIM, NC, T>(kp, R, dst, os, ocs, oo);
+    q2_scatter<DIM, NC, T>(kp, L, R, dst, os, ocs, oo);
   }
 } // namespace hps

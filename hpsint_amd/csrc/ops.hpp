@@ -118,7 +118,8 @@ namespace hps
     return kp;
   }
 
-  // workgroups of all colours (the cache holds n_fields * 64 entries each)
+  // workgroups of all colours (the cache holds n_fields * cpw * (p+1)^dim
+  // entries each)
   inline int64_t q2_workgroups(const Q2Params &kp, int dim, int cpw)
   {
     int64_t wg = 0;
