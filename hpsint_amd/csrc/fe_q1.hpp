@@ -185,39 +185,27 @@ namespace hps
   // transpose of interp_vg: nodal contribution of (v, g) given at q-points,
   // g already multiplied by signed_scales(corner, inv_h, JxW): the partner
   // across direction d holds the opposite sign, so s_d (g + g_partner) is
-  // the difference of the signed values
+  // the difference of the signed values.  Where a level combines a value
+  // row and a derivative row, the partner's whole contribution
+  // (sb * value - derivative) is exchanged as one quantity.
   template <int DIM, typename T>
   __device__ __forceinline__ T integrate_vg(T v, const T *g)
   {
     const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
     if constexpr (DIM == 2)
       {
-        const T pv  = xshfl<2>(v);
-        const T pgy = xshfl<2>(g[1]);
-        const T pgx = xshfl<2>(g[0]);
-        const T U1  = sa * v + (sb * pv + (g[1] - pgy));
-        const T Ux  = sa * g[0] + sb * pgx;
-        const T pU1 = xshfl<1>(U1);
-        const T pUx = xshfl<1>(Ux);
-        return sa * U1 + (sb * pU1 + (Ux - pUx));
+        const T U1 = (sa * v + g[1]) + xshfl<2>(sb * v - g[1]);
+        const T Ux = sa * g[0] + sb * xshfl<2>(g[0]);
+        return (sa * U1 + Ux) + xshfl<1>(sb * U1 - Ux);
       }
     else
       {
-        const T pv  = xshfl<1>(v);
-        const T pgx = xshfl<1>(g[0]);
-        const T pgy = xshfl<1>(g[1]);
-        const T pgz = xshfl<1>(g[2]);
-        const T T1  = sa * v + (sb * pv + (g[0] - pgx));
-        const T Ty  = sa * g[1] + sb * pgy;
-        const T Tz  = sa * g[2] + sb * pgz;
-        const T pT1 = xshfl<2>(T1);
-        const T pTy = xshfl<2>(Ty);
-        const T pTz = xshfl<2>(Tz);
-        const T U1  = sa * T1 + (sb * pT1 + (Ty - pTy));
-        const T Uz  = sa * Tz + sb * pTz;
-        const T pU1 = xshfl<4>(U1);
-        const T pUz = xshfl<4>(Uz);
-        return sa * U1 + (sb * pU1 + (Uz - pUz));
+        const T T1 = (sa * v + g[0]) + xshfl<1>(sb * v - g[0]);
+        const T Ty = sa * g[1] + sb * xshfl<1>(g[1]);
+        const T Tz = sa * g[2] + sb * xshfl<1>(g[2]);
+        const T U1 = (sa * T1 + Ty) + xshfl<2>(sb * T1 - Ty);
+        const T Uz = sa * Tz + sb * xshfl<2>(Tz);
+        return (sa * U1 + Uz) + xshfl<4>(sb * U1 - Uz);
       }
   }
 
