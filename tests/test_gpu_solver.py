@@ -80,3 +80,34 @@ def test_time_loop_two_particles_10_steps():
     assert r <= 2e-8 * r0, (r, r0)
     # the neck grows: the field moves
     assert np.linalg.norm(u - u0) > 0
+
+
+@pytest.mark.parametrize("tensorial", [False, True])
+@pytest.mark.parametrize("inner_fp32", [False, True])
+def test_q2_newton_step_satisfies_oracle_residual(tensorial, inner_fp32):
+    """the two-particle step on FE_Q(2) (80x40 cells: the nodes of the 160x80
+    Q1 mesh), scalar and tensorial mobility"""
+    pk = packing.two_particles_2d()
+    cells = (80, 40)
+    mesh = CartesianMesh(cells, pk.lower, pk.upper)
+    mob = MobilityParams(tensorial=tensorial)
+    op = SinteringOperatorGeneric(mesh, 2, FreeEnergyParams(), mob, fe_degree=2)
+    u0 = packing.initial_fields(pk, (160, 80))
+    dt = 1e-3
+    op.set_time_weights(1.0 / dt, (-1.0 / dt,))
+    solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
+                          preconditioner="InverseComponentBlockDiagonalMatrix",
+                          inner_fp32=inner_fp32)
+    old = torch.from_numpy(u0).cuda()
+    x = old.clone()
+    st = solver.solve(x, [old])
+    assert st.converged and st.newton_iterations >= 1
+    fe = FreeEnergyParams()
+    p = po.make_params(2, 2, cells, mesh.cell_size, 4, tensorial=tensorial, A=fe.A, B=fe.B,
+                       kappa_c=fe.kappa_c, kappa_p=fe.kappa_p, Mvol=mob.Mvol, Mvap=mob.Mvap,
+                       Msurf=mob.Msurf, Mgb=mob.Mgb, L=mob.L, w0=1.0 / dt,
+                       w_hist=(-1.0 / dt,))
+    xs = x.cpu().numpy()
+    r0 = np.linalg.norm(po.residual(p, 2, u0, [u0]))
+    r = np.linalg.norm(po.residual(p, 2, xs, [u0]))
+    assert r <= 2e-8 * r0, (r, r0)
