@@ -390,6 +390,93 @@ namespace hps
         }
   }
 
+  // the same inversion for larger blocks, whose two matrices do not fit in
+  // one lane's registers: a group of 16 lanes per node block, lane i holds
+  // row i of the block and of the inverse.  Partial pivoting by the largest
+  // magnitude among the rows not yet used (lowest row on ties); pivot rows
+  // are not moved, so the lane that pivots column j ends with row j of the
+  // inverse.  The normalised pivot row reaches the group through LDS.
+  template <int NC>
+  __global__ void __launch_bounds__(64)
+    k_invert_blocks_rows(int64_t n_nodes, const double *__restrict__ blocks,
+                         double *__restrict__ inv64, float *__restrict__ inv32,
+                         int *__restrict__ singular)
+  {
+    static_assert(NC <= 16, "one row per lane of a 16-lane group");
+    __shared__ double piv[4][2 * NC];
+    const int     grp  = threadIdx.x >> 4, i = threadIdx.x & 15;
+    const int64_t node = (int64_t)blockIdx.x * 4 + grp;
+    const bool    live = node < n_nodes && i < NC;
+    double        a[NC], b[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      {
+        a[k] = live ? blocks[(node * NC + i) * NC + k] : 0.0;
+        b[k] = k == i ? 1.0 : 0.0;
+      }
+    bool used = !live; // rows of absent lanes never pivot
+    bool sing = false;
+    int  col  = 0;     // the column this lane pivoted
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      {
+        double v = used ? -1.0 : fabs(a[j]);
+        int    r = i;
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1)
+          {
+            const double ov = __shfl_xor(v, m, 16);
+            const int    orr = __shfl_xor(r, m, 16);
+            if (ov > v || (ov == v && orr < r))
+              {
+                v = ov;
+                r = orr;
+              }
+          }
+        sing         = sing || !(v > 0.0);
+        const bool me = i == r;
+        if (me)
+          {
+            const double hr = sing ? 0.0 : 1.0 / a[j];
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+              {
+                a[k] *= hr;
+                b[k] *= hr;
+                piv[grp][k]      = a[k];
+                piv[grp][NC + k] = b[k];
+              }
+            used = true;
+            col  = j;
+          }
+        __syncthreads();
+        if (!me && live)
+          {
+            const double f = a[j];
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+              {
+                a[k] -= f * piv[grp][k];
+                b[k] -= f * piv[grp][NC + k];
+              }
+          }
+        __syncthreads();
+      }
+    if (sing)
+      {
+        if (i == 0 && node < n_nodes)
+          atomicAdd(singular, 1);
+        return;
+      }
+    if (live)
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+        {
+          inv64[(node * NC + col) * NC + k] = b[k];
+          inv32[(node * NC + col) * NC + k] = (float)b[k];
+        }
+  }
+
   template <int NC, typename T>
   __global__ void k_block_apply(int64_t n_nodes, const T *__restrict__ inv,
                                 const T *__restrict__ src, T *__restrict__ dst)
@@ -591,8 +678,14 @@ namespace hps
   bool invert_blocks(hipStream_t s, int64_t n_nodes, int nc, const double *blocks,
                      double *inv64, float *inv32, int *singular)
   {
-    const int g = (int)((n_nodes + 63) / 64);
-#define HPS_CALL(N) k_invert_blocks<N><<<g, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
+    // up to 6 components both matrices stay in one lane's registers
+    const int g  = (int)((n_nodes + 63) / 64);
+    const int g4 = (int)((n_nodes + 3) / 4);
+#define HPS_CALL(N)                                                                          \
+  if constexpr (N <= 6)                                                                      \
+    k_invert_blocks<N><<<g, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular);            \
+  else                                                                                       \
+    k_invert_blocks_rows<N><<<g4, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
     return true;

@@ -131,6 +131,15 @@ namespace hps
     r[1]    = sel3(t, C::e_b, T(0), -C::e_b);
     r[2]    = sel3(t, C::e_c, T(1), -C::e_a);
   }
+  // column t of L': out[q] = L'[q][t] (derivative of basis t at point q)
+  template <typename T>
+  __device__ __forceinline__ void q2_e_col(int t, T *r)
+  {
+    using C = Q2Coef<T>;
+    r[0]    = sel3(t, C::e_a, C::e_b, C::e_c);
+    r[1]    = sel3(t, T(-1), T(0), T(1));
+    r[2]    = sel3(t, -C::e_c, -C::e_b, -C::e_a);
+  }
   template <typename T>
   __device__ __forceinline__ T q2_w(int t)
   {
@@ -851,6 +860,47 @@ namespace hps
   // per direction m of (value, d/dx, ...) with a unit input, and the node
   // sums over the Gauss points run through LDS.  The result goes to
   // dst[node * os + a * ocs + oo].
+  // ------------------------------------------------ diagonal / node blocks
+  // Node block B[j][a][b] = sum_q JxW psi_j,m'(q) M_{a m', b m}(q) psi_j,m(q)
+  // (psi_j,0 = phi_j, psi_j,d+1 = d phi_j / dx_d; M: the quadrature-point
+  // tangent from input (component b, value or gradient d) to output
+  // (component a, value or gradient d')).  M is sparse: value rows read
+  // values only; the c gradient row reads the values and the gradients of
+  // c and mu; the mu and eta_i gradient rows are -kappa_c grad c and
+  // L kappa_p grad eta_i.  Each nonzero term is a tensor product of 1D
+  // weights w_q f(x_q) g(x_q) with f, g in {s_j, s_j'/h} per direction, so
+  // the q-sum is three sum-factorisation passes over the field M(q); terms
+  // with the same 1D weights share a field.  Rows (NR = n_c + 12):
+  //   r < n_c        phi phi           M_{r 0, b 0}          -> B[.][r][b]
+  //   n_c + d        phi' phi (dir d)  M_{0 d+1, b 0}        -> B[.][0][b]
+  //   n_c + 3 + d    phi' phi' (d, d)  M_{0 d+1, b d+1}      -> B[.][0][b]
+  //   n_c + 6 + p    phi' phi' (d, e)  M_{0 d+1, b e+1} + M_{0 e+1, b d+1}
+  //   n_c + 9 + d    phi' phi' (d, d)  -kappa_c (b = 0) -> B[.][1][0],
+  //                                    L kappa_p (b >= 2) -> B[.][b][b]
+  // The diagonal (column < 0) uses the same rows with the mass field of
+  // row b taken from input b, the c-row fields from input c, and the
+  // stiffness rows with field 1 scaled by L kappa_p for every b >= 2.
+  template <int DIM>
+  struct Q2ProbeRows
+  {
+    // 1D weight type of row r along direction d: 0 phi phi, 1 phi' phi,
+    // 2 phi' phi'
+    __device__ static constexpr int type(int nc, int r, int d)
+    {
+      if (r < nc)
+        return 0;
+      const int k = r - nc;
+      if (k < 3)
+        return k == d ? 1 : 0;
+      if (k < 6 || k >= 9)
+        return (k % 3) == d ? 2 : 0;
+      // pairs (0,1), (0,2), (1,2)
+      const int p = k - 6;
+      const int e0 = p == 2 ? 1 : 0, e1 = p == 0 ? 1 : 2;
+      return (d == e0 || d == e1) ? 1 : 0;
+    }
+  };
+
   template <int DIM, int G, bool TENS>
   __global__ void __launch_bounds__(64)
     q2_probe(const Q2Params kp, const double *__restrict__ cache, double *__restrict__ dst,
@@ -859,134 +909,180 @@ namespace hps
     using CL         = CacheLayout<DIM, G, TENS>;
     constexpr int NC = G + 2;
     constexpr int NF = CL::n_fields;
+    constexpr int NR = NC + 12;
     using LN         = Q2Lane<DIM>;
-    constexpr int NP = LN::NP;
     using T          = double;
+    using RW         = Q2ProbeRows<DIM>;
 
-    __shared__ T X[LN::CPW * NP * NP]; // [cell][node][q]
-    __shared__ T R[NC * 64];
+    __shared__ T A[NR * 64];
+    __shared__ T B[NR * 64];
 
-    const LN L(kp);
-    T        lin[NF];
+    const LN  L(kp);
+    const int l    = L.l;
+    const int b    = column;
+    const T   kc   = -T(kp.pp.kappa_c);
+    const T   lkp  = T(kp.pp.L) * T(kp.pp.kappa_p);
+
+    // 1) fields of the rows at this lane's Gauss point, into A
     {
-      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LN::LP + L.l;
+      T lin[NF];
+      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LN::LP + l;
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         lin[f] = L.valid ? cl[f * LN::LP] : T(0);
+      const T zero[3] = {T(0), T(0), T(0)};
+      // c-row gradient outputs for the inputs of component cb: value (m = 0)
+      // and gradient d (m = d + 1)
+      auto c_rows = [&](int cb) {
+        T v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          v[c] = c == cb ? T(1) : T(0);
+        T gm[DIM + 1][DIM];
+        tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, zero, zero, gm[0]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          v[c] = T(0);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+          {
+            T u[3] = {T(0), T(0), T(0)};
+            u[d]   = T(1);
+            if (cb <= 1)
+              tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, cb == 0 ? u : zero,
+                                           cb == 1 ? u : zero, gm[d + 1]);
+            else
+#pragma unroll
+              for (int e = 0; e < DIM; ++e)
+                gm[d + 1][e] = T(0);
+          }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          {
+            A[(NC + d) * 64 + l]     = d < DIM ? gm[0][d] : T(0);
+            A[(NC + 3 + d) * 64 + l] = d < DIM ? gm[d + 1][d] : T(0);
+          }
+        // pairs (0,1), (0,2), (1,2)
+        A[(NC + 6) * 64 + l] = gm[2][0] + gm[1][1];
+        A[(NC + 7) * 64 + l] = DIM == 3 ? gm[3 % (DIM + 1)][0] + gm[1][2 % DIM] : T(0);
+        A[(NC + 8) * 64 + l] = DIM == 3 ? gm[3 % (DIM + 1)][1] + gm[2][2 % DIM] : T(0);
+      };
+      if (b >= 0)
+        {
+          T v[NC], vr[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            v[c] = c == b ? T(1) : T(0);
+          tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            A[c * 64 + l] = vr[c];
+          c_rows(b);
+          const T st = b == 0 ? kc : (b >= 2 ? lkp : T(0));
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            A[(NC + 9 + d) * 64 + l] = d < DIM ? st : T(0);
+        }
+      else
+        {
+#pragma unroll 1
+          for (int cb = 0; cb < NC; ++cb)
+            {
+              T v[NC], vr[NC];
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                v[c] = c == cb ? T(1) : T(0);
+              tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+              T x = T(0);
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                if (c == cb)
+                  x = vr[c];
+              A[cb * 64 + l] = x;
+            }
+          c_rows(0);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            A[(NC + 9 + d) * 64 + l] = d < DIM ? T(1) : T(0);
+        }
     }
-    T jxw = T(kp.h_prod);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      jxw *= q2_w<T>(L.t[d]);
-    // shape values / derivatives of the 1D bases at this lane's Gauss point
-    T sr[DIM][3], er[DIM][3];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      {
-        q2_s_row(L.t[d], sr[d]);
-        q2_e_row(L.t[d], er[d]);
-#pragma unroll
-        for (int n = 0; n < 3; ++n)
-          er[d][n] *= T(kp.inv_h[d]);
-      }
-    const bool in_cell = L.s < LN::CPW;
+    __syncthreads();
 
+    // 2) q -> node sums, one pass per direction over all rows
+    //    (A -> B -> A in 3D, the last pass into registers)
+    T res[NR];
+    {
+      T K[3][3]; // [type][q] for the current direction
+      auto coefs = [&](int d) {
+        T sc[3], ec[3];
+        q2_s_col(L.t[d], sc);
+        q2_e_col(L.t[d], ec);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          {
+            const T w = q2_w<T>(q);
+            const T e = ec[q] * T(kp.inv_h[d]);
+            K[0][q]   = w * sc[q] * sc[q];
+            K[1][q]   = w * sc[q] * e;
+            K[2][q]   = w * e * e;
+          }
+      };
+      coefs(0);
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        B[r * 64 + l] = q2_pass(A + r * 64, L.base(0), 1, K[RW::type(NC, r, 0)]);
+      __syncthreads();
+      coefs(1);
+      if constexpr (DIM == 3)
+        {
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+            A[r * 64 + l] = q2_pass(B + r * 64, L.base(1), 3, K[RW::type(NC, r, 1)]);
+          __syncthreads();
+          coefs(2);
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+            res[r] = q2_pass(A + r * 64, L.base(2), 9, K[RW::type(NC, r, 2)]);
+        }
+      else
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          res[r] = q2_pass(B + r * 64, L.base(1), 3, K[RW::type(NC, r, 1)]);
+    }
+
+    // 3) rows to outputs
     T acc[NC];
 #pragma unroll
     for (int a = 0; a < NC; ++a)
-      acc[a] = T(0);
-
-    for (int b = 0; b < NC; ++b)
+      acc[a] = res[a];
+    T c_sum = T(0), st_sum = T(0);
+#pragma unroll
+    for (int r = NC; r < NC + 9; ++r)
+      c_sum += res[r];
+#pragma unroll
+    for (int r = NC + 9; r < NR; ++r)
+      st_sum += res[r];
+    const T hp = T(kp.h_prod);
+    if (b >= 0)
       {
-        if (column >= 0 && b != column)
-          continue;
-        for (int m = 0; m <= DIM; ++m)
-          {
-            T v[NC], g[NC][DIM], vr[NC], gr[NC][DIM];
+        acc[0] += c_sum;
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-              {
-                v[c] = (c == b && m == 0) ? T(1) : T(0);
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  g[c][d] = (c == b && m == d + 1) ? T(1) : T(0);
-              }
-            tangent_qp<DIM, G, TENS>(kp.pp, lin, v, g, vr, gr);
-            for (int a = 0; a < NC; ++a)
-              {
-                if (column < 0 && a != b)
-                  continue;
-                // row a of the output, selected without dynamic register
-                // indexing
-                T va = T(0), ga[DIM];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  ga[d] = T(0);
-#pragma unroll
-                for (int c = 0; c < NC; ++c)
-                  if (c == a)
-                    {
-                      va = vr[c];
-#pragma unroll
-                      for (int d = 0; d < DIM; ++d)
-                        ga[d] = gr[c][d];
-                    }
-                // X[cell][j][q] = JxW phi_j,m(q) (vr_a phi_j(q) + gr_a . grad phi_j(q))
-                // (a rolled loop with selects: unrolled, the compiler keeps
-                // all 27 shape tuples live and spills)
-                if (in_cell)
-#pragma unroll 1
-                  for (int j = 0; j < NP; ++j)
-                    {
-                      const int jt[3] = {j % 3, (j / 3) % 3, j / 9};
-                      T         sv[DIM], ev[DIM];
-#pragma unroll
-                      for (int d = 0; d < DIM; ++d)
-                        {
-                          sv[d] = sel3(jt[d], sr[d][0], sr[d][1], sr[d][2]);
-                          ev[d] = sel3(jt[d], er[d][0], er[d][1], er[d][2]);
-                        }
-                      T phi = T(1);
-#pragma unroll
-                      for (int d = 0; d < DIM; ++d)
-                        phi *= sv[d];
-                      T w  = va * phi;
-                      T pm = m == 0 ? phi : T(0);
-#pragma unroll
-                      for (int d = 0; d < DIM; ++d)
-                        {
-                          T gd = ev[d];
-#pragma unroll
-                          for (int e = 0; e < DIM; ++e)
-                            if (e != d)
-                              gd *= sv[e];
-                          w += ga[d] * gd;
-                          if (m == d + 1)
-                            pm = gd;
-                        }
-                      X[(L.s * NP + j) * NP + L.p] = jxw * pm * w;
-                    }
-                __syncthreads();
-                if (in_cell)
-                  {
-                    T        s  = T(0);
-                    const T *xr = &X[(L.s * NP + L.p) * NP];
-#pragma unroll
-                    for (int q = 0; q < NP; ++q)
-                      s += xr[q];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c)
-                      if (c == a)
-                        acc[c] += s;
-                  }
-                __syncthreads();
-              }
-          }
+        for (int a = 1; a < NC; ++a)
+          if (a == (b == 0 ? 1 : b) && b != 1)
+            acc[a] += st_sum;
       }
+    else
+      {
+        acc[0] += c_sum;
+#pragma unroll
+        for (int a = 2; a < NC; ++a)
+          acc[a] += lkp * st_sum;
+      }
+    __syncthreads();
 #pragma unroll
     for (int a = 0; a < NC; ++a)
-      R[a * 64 + L.l] = acc[a];
+      A[a * 64 + l] = hp * acc[a];
     __syncthreads();
-    q2_scatter<DIM, NC, T>(kp, L, R, dst, os, ocs, oo);
+    q2_scatter<DIM, NC, T>(kp, L, A, dst, os, ocs, oo);
   }
 } // namespace hps
