@@ -591,24 +591,26 @@ static int precond_update(hps_operator op, int kind, bool fp32)
           HPS_HIP(hipMalloc(&op->inv_blocks64, nn * nc * nc * sizeof(double)));
           HPS_HIP(hipMalloc(&op->inv_blocks32, nn * nc * nc * sizeof(float)));
         }
+      // the assembled blocks are stored per node column by column
+      // (blocks[node][column][row]): a block column is a contiguous run
       for (int j = 0; j < nc; ++j)
         {
           if (op->q2)
-            op->q2->probe(q2_params(op), op->cache64, op->blocks, j, nc * nc, nc, j,
+            op->q2->probe(q2_params(op), op->cache64, op->blocks, j, nc * nc, 1, j * nc,
                           op->stream);
           else if (fp32)
             op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_blocks32,
-                             (float *)op->part, j, nc * nc, nc, j, op->stream);
+                             (float *)op->part, j, nc * nc, 1, j * nc, op->stream);
           else
             op->ops->probe(op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc,
-                           nc, j, op->stream);
+                           1, j * nc, op->stream);
           HPS_LAUNCHED();
           if (fp32)
             {
-              if (int r = finish_fixup<float>(op, op->inv_blocks32, nc * nc, nc, j))
+              if (int r = finish_fixup<float>(op, op->inv_blocks32, nc * nc, 1, j * nc))
                 return r;
             }
-          else if (int r = finish_fixup<double>(op, op->blocks, nc * nc, nc, j))
+          else if (int r = finish_fixup<double>(op, op->blocks, nc * nc, 1, j * nc))
             return r;
         }
       if (fp32)
@@ -691,8 +693,9 @@ extern "C" int hps_precond_get_blocks(hps_operator op, double *blocks)
     return r;
   if (!op->blocks)
     return fail(HPS_ERR_NOT_INITIALIZED, "no node blocks computed");
-  HPS_HIP(hipMemcpyAsync(blocks, op->blocks, op->n_nodes * op->nc * op->nc * sizeof(double),
-                         hipMemcpyDeviceToDevice, op->stream));
+  // stored column by column per node; returned row by row
+  hps::transpose_blocks(op->stream, op->n_nodes, op->nc, op->blocks, blocks);
+  HPS_LAUNCHED();
   return HPS_OK;
 }
 

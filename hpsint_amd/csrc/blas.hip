@@ -327,7 +327,7 @@ namespace hps
 #pragma unroll
       for (int j = 0; j < NC; ++j)
         {
-          a[i][j] = blocks[(node * NC + i) * NC + j];
+          a[i][j] = blocks[(node * NC + j) * NC + i]; // stored column by column
           b[i][j] = i == j ? 1.0 : 0.0;
         }
 #pragma unroll
@@ -411,7 +411,7 @@ namespace hps
 #pragma unroll
     for (int k = 0; k < NC; ++k)
       {
-        a[k] = live ? blocks[(node * NC + i) * NC + k] : 0.0;
+        a[k] = live ? blocks[(node * NC + k) * NC + i] : 0.0; // stored column by column
         b[k] = k == i ? 1.0 : 0.0;
       }
     bool used = !live; // rows of absent lanes never pivot
@@ -876,5 +876,25 @@ namespace hps
   {
     k_constraint_identity_blocks<<<grid_for(n * nc * nc, 256), 256, 0, s>>>(n, nc, nodes,
                                                                              blocks);
+  }
+
+  // node blocks stored column by column -> row by row
+  __global__ void k_transpose_blocks(int64_t n_nodes, int nc, const double *__restrict__ in,
+                                     double *__restrict__ out)
+  {
+    const int64_t per   = (int64_t)nc * nc;
+    const int64_t total = n_nodes * per;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x)
+      {
+        const int64_t node = e / per;
+        const int     r    = (int)(e - node * per);
+        const int     a = r / nc, b = r - (r / nc) * nc;
+        out[e]             = in[node * per + (int64_t)b * nc + a];
+      }
+  }
+  void transpose_blocks(hipStream_t s, int64_t n_nodes, int nc, const double *in, double *out)
+  {
+    k_transpose_blocks<<<grid_for(n_nodes * nc * nc, 256), 256, 0, s>>>(n_nodes, nc, in, out);
   }
 } // namespace hps

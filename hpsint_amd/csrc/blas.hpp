@@ -59,4 +59,6 @@ namespace hps
                       double value);
   void constraint_identity_blocks(hipStream_t, int64_t n, int nc, const int64_t *nodes,
                                   double *blocks);
+  // node blocks stored column by column -> row by row (in != out)
+  void transpose_blocks(hipStream_t, int64_t n_nodes, int nc, const double *in, double *out);
 } // namespace hps

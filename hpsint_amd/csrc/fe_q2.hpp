@@ -408,9 +408,9 @@ namespace hps
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       r[c] = buf[c * 64 + L.l];
-    if (os == NC && ocs == 1 && oo == 0)
+    if (ocs == 1)
       {
-        T *d = dst + n * NC;
+        T *d = dst + n * os + oo;
         if (!first)
           {
             T o[NC];
