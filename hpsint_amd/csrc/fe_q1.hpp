@@ -642,6 +642,157 @@ namespace hps
   // compute_inverse_diagonal, S/operator_base.h:364-389).
   // T = float assembles from the FP32 cache in FP32 arithmetic (the
   // preconditioner of the FP32 inner solve)
+  // Node-block column (or diagonal) entries of this lane's corner node in
+  // its cell: B[c][a][b] = sum_q JxW psi_c,m'(q) M_{a m', b m}(q) psi_c,m(q)
+  // with M the quadrature-point tangent from input (component b, value or
+  // gradient m) to output (component a, value or gradient m').  M is
+  // sparse: value rows read values only; the c gradient row reads the
+  // values and the gradients of c and mu; the mu and eta_i gradient rows
+  // are -kappa_c grad c and L kappa_p grad eta_i.  Each nonzero term is a
+  // tensor product of 1D weights (s s, s s', s' s' of the node's basis at
+  // the Gauss points), so the sum over the cell's q-points is one
+  // butterfly per direction over the fields
+  //   r < n_c      mass        M_{r 0, b 0}                   -> row r
+  //   n_c + d      s' s  (d)   M_{0 d+1, b 0}                 -> row 0
+  //   n_c + 3 + d  s' s' (d)   M_{0 d+1, b d+1}               -> row 0
+  //   n_c + 6 + p  s' s' (d,e) M_{0 d+1, b e+1} + M_{0 e+1, b d+1} -> row 0
+  //   n_c + 9 + d  s' s' (d)   -kappa_c (b = 0, row 1), L kappa_p (b >= 2)
+  // (diagonal, column < 0: mass row b from input b, the c-row fields from
+  // input c, the stiffness fields 1, scaled by L kappa_p for b >= 2)
+  template <int DIM, int G, bool TENS, typename T>
+  __device__ __forceinline__ void q1_probe_cell(const KParams &kp, int corner, const T *lin,
+                                                int column, const T *inv_h, T *out)
+  {
+    constexpr int NC = G + 2;
+    constexpr int NR = NC + 12;
+    const T       sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    const T       kc = -T(kp.pp.kappa_c), lkp = T(kp.pp.L) * T(kp.pp.kappa_p);
+    const int     b  = column;
+    T             F[NR];
+    const T       zero[3] = {T(0), T(0), T(0)};
+    {
+      // c-row gradient outputs for the inputs of component cb
+      const int cb = b >= 0 ? b : 0;
+      T         v[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        v[c] = c == cb ? T(1) : T(0);
+      T gm[DIM + 1][DIM];
+      tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, zero, zero, gm[0]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        v[c] = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        {
+          T u[3] = {T(0), T(0), T(0)};
+          u[d]   = T(1);
+          if (cb <= 1)
+            tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, cb == 0 ? u : zero, cb == 1 ? u : zero,
+                                         gm[d + 1]);
+          else
+#pragma unroll
+            for (int e = 0; e < DIM; ++e)
+              gm[d + 1][e] = T(0);
+        }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        {
+          F[NC + d]     = d < DIM ? gm[0][d] : T(0);
+          F[NC + 3 + d] = d < DIM ? gm[d + 1][d] : T(0);
+        }
+      F[NC + 6] = gm[2][0] + gm[1][1];
+      F[NC + 7] = DIM == 3 ? gm[3 % (DIM + 1)][0] + gm[1][2 % DIM] : T(0);
+      F[NC + 8] = DIM == 3 ? gm[3 % (DIM + 1)][1] + gm[2][2 % DIM] : T(0);
+    }
+    if (b >= 0)
+      {
+        T v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          v[c] = c == b ? T(1) : T(0);
+        tangent_values<DIM, G, TENS>(kp.pp, lin, v, F);
+        const T st = b == 0 ? kc : (b >= 2 ? lkp : T(0));
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          F[NC + 9 + d] = d < DIM ? st : T(0);
+      }
+    else
+      {
+#pragma unroll
+        for (int cb = 0; cb < NC; ++cb)
+          {
+            T v[NC], vr[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              v[c] = c == cb ? T(1) : T(0);
+            tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+            F[cb] = vr[cb];
+          }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          F[NC + 9 + d] = d < DIM ? T(1) : T(0);
+      }
+    // q -> node: one butterfly per direction; the lane's bit d stands for
+    // the Gauss point before and for the node after level d
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        const bool hi = (corner >> d) & 1;
+        const T    e  = hi ? inv_h[d] : -inv_h[d]; // s' of the node's basis
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          {
+            // weight type of field r along d: 0 s s, 1 s' s, 2 s' s'
+            int ty = 0;
+            if (r >= NC)
+              {
+                const int k = r - NC;
+                if (k < 3)
+                  ty = k == d ? 1 : 0;
+                else if (k < 6 || k >= 9)
+                  ty = (k % 3) == d ? 2 : 0;
+                else
+                  {
+                    const int p = k - 6, e0 = p == 2 ? 1 : 0, e1 = p == 0 ? 1 : 2;
+                    ty = (d == e0 || d == e1) ? 1 : 0;
+                  }
+              }
+            const T own = F[r];
+            const T par = d == 0 ? xshfl<1>(own) : (d == 1 ? xshfl<2>(own) : xshfl<4>(own));
+            if (ty == 0)
+              F[r] = (sa * sa) * own + (sb * sb) * par;
+            else if (ty == 1)
+              F[r] = e * (sa * own + sb * par);
+            else
+              F[r] = (e * e) * (own + par);
+          }
+      }
+    const T jxw = T(kp.jxw);
+    T       c_sum = T(0), st_sum = T(0);
+#pragma unroll
+    for (int r = NC; r < NC + 9; ++r)
+      c_sum += F[r];
+#pragma unroll
+    for (int r = NC + 9; r < NR; ++r)
+      st_sum += F[r];
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+      {
+        T x = F[a];
+        if (a == 0)
+          x += c_sum;
+        if (b >= 0)
+          {
+            if (b != 1 && a == (b == 0 ? 1 : b))
+              x += st_sum;
+          }
+        else if (a >= 2)
+          x += lkp * st_sum;
+        out[a] = jxw * x;
+      }
+  }
+
   template <int DIM, int TX, int TY, int G, bool TENS, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
     q1_probe(const KParams kp, const T *__restrict__ cache, T *__restrict__ dst,
@@ -660,8 +811,6 @@ namespace hps
     const TL     t(kp);
     const int    tid = t.tid;
     const T      inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
-    const T      jxw      = T(kp.jxw);
-    const T      sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
 
     T carry[ITEMS];
 #pragma unroll
@@ -678,60 +827,7 @@ namespace hps
           lin[f] = cl[f * NT];
 
         T mine[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          mine[c] = T(0);
-
-        for (int j = 0; j < NC; ++j)
-          {
-            if (column >= 0 && j != column)
-              continue;
-            for (int kk = 0; kk < NV; ++kk)
-              {
-                // shape value / gradient of local node kk at this lane's q-point
-                T S = T(1), Gd[DIM];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  {
-                    const int qd = (t.corner >> d) & 1, nd = (kk >> d) & 1;
-                    S *= (qd == nd) ? sa : sb;
-                  }
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  {
-                    T g = (((kk >> d) & 1) ? T(1) : T(-1)) * inv_h[d];
-#pragma unroll
-                    for (int e = 0; e < DIM; ++e)
-                      if (e != d)
-                        g *= (((t.corner >> e) & 1) == ((kk >> e) & 1)) ? sa : sb;
-                    Gd[d] = g;
-                  }
-                T v[NC], g[NC][DIM];
-#pragma unroll
-                for (int c = 0; c < NC; ++c)
-                  {
-                    v[c] = c == j ? S : T(0);
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d)
-                      g[c][d] = c == j ? Gd[d] : T(0);
-                  }
-                T vr[NC], gr[NC][DIM];
-                tangent_qp<DIM, G, TENS>(kp.pp, lin, v, g, vr, gr);
-#pragma unroll
-                for (int i = 0; i < NC; ++i)
-                  {
-                    if (column < 0 && i != j)
-                      continue;
-                    T s = S * vr[i];
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d)
-                      s += Gd[d] * gr[i][d];
-                    s = cell_sum<DIM>(jxw * s);
-                    if (t.corner == kk)
-                      mine[i] = s;
-                  }
-              }
-          }
+        q1_probe_cell<DIM, G, TENS>(kp, t.corner, lin, column, inv_h, mine);
         T *sl = &slots[t.corner][t.pn_lane * NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
