@@ -593,12 +593,14 @@ static int precond_update(hps_operator op, int kind, bool fp32)
         }
       // the assembled blocks are stored per node column by column
       // (blocks[node][column][row]): a block column is a contiguous run
-      for (int j = 0; j < nc; ++j)
+      if (op->q2) // all block columns in one pass over the cache
         {
-          if (op->q2)
-            op->q2->probe(q2_params(op), op->cache64, op->blocks, j, nc * nc, 1, j * nc,
-                          op->stream);
-          else if (fp32)
+          op->q2->probe(q2_params(op), op->cache64, op->blocks, -2, nc * nc, 1, 0, op->stream);
+          HPS_LAUNCHED();
+        }
+      for (int j = 0; j < nc && !op->q2; ++j)
+        {
+          if (fp32)
             op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_blocks32,
                              (float *)op->part, j, nc * nc, 1, j * nc, op->stream);
           else

@@ -919,170 +919,189 @@ namespace hps
 
     const LN  L(kp);
     const int l    = L.l;
-    const int b    = column;
     const T   kc   = -T(kp.pp.kappa_c);
     const T   lkp  = T(kp.pp.L) * T(kp.pp.kappa_p);
 
-    // 1) fields of the rows at this lane's Gauss point, into A
-    {
-      T lin[NF];
-      const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LN::LP + l;
+    // entries of block column b (b < 0: the diagonal) of this lane's node,
+    // left in A[a * 64 + lane]
+    auto column_entries = [&](const int b) {
+      // 1) fields of the rows at this lane's Gauss point, into A
+      {
+        // (read again for every column, from the caches: the opaque offset
+        // keeps the loads inside the column loop, so the fields are not
+        // held in registers across it)
+        int off = 0;
+        asm volatile("" : "+v"(off));
+        T        lin[NF];
+        const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LN::LP + l + off;
 #pragma unroll
-      for (int f = 0; f < NF; ++f)
-        lin[f] = L.valid ? cl[f * LN::LP] : T(0);
-      const T zero[3] = {T(0), T(0), T(0)};
-      // c-row gradient outputs for the inputs of component cb: value (m = 0)
-      // and gradient d (m = d + 1)
-      auto c_rows = [&](int cb) {
-        T v[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          v[c] = c == cb ? T(1) : T(0);
-        T gm[DIM + 1][DIM];
-        tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, zero, zero, gm[0]);
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          v[c] = T(0);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          {
-            T u[3] = {T(0), T(0), T(0)};
-            u[d]   = T(1);
-            if (cb <= 1)
-              tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, cb == 0 ? u : zero,
-                                           cb == 1 ? u : zero, gm[d + 1]);
-            else
-#pragma unroll
-              for (int e = 0; e < DIM; ++e)
-                gm[d + 1][e] = T(0);
-          }
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          {
-            A[(NC + d) * 64 + l]     = d < DIM ? gm[0][d] : T(0);
-            A[(NC + 3 + d) * 64 + l] = d < DIM ? gm[d + 1][d] : T(0);
-          }
-        // pairs (0,1), (0,2), (1,2)
-        A[(NC + 6) * 64 + l] = gm[2][0] + gm[1][1];
-        A[(NC + 7) * 64 + l] = DIM == 3 ? gm[3 % (DIM + 1)][0] + gm[1][2 % DIM] : T(0);
-        A[(NC + 8) * 64 + l] = DIM == 3 ? gm[3 % (DIM + 1)][1] + gm[2][2 % DIM] : T(0);
-      };
-      if (b >= 0)
-        {
-          T v[NC], vr[NC];
+        for (int f = 0; f < NF; ++f)
+          lin[f] = L.valid ? cl[f * LN::LP] : T(0);
+        const T zero[3] = {T(0), T(0), T(0)};
+        // c-row gradient outputs for the inputs of component cb: value (m = 0)
+        // and gradient d (m = d + 1)
+        auto c_rows = [&](int cb) {
+          T v[NC];
 #pragma unroll
           for (int c = 0; c < NC; ++c)
-            v[c] = c == b ? T(1) : T(0);
-          tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+            v[c] = c == cb ? T(1) : T(0);
+          T gm[DIM + 1][DIM];
+          tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, zero, zero, gm[0]);
 #pragma unroll
           for (int c = 0; c < NC; ++c)
-            A[c * 64 + l] = vr[c];
-          c_rows(b);
-          const T st = b == 0 ? kc : (b >= 2 ? lkp : T(0));
+            v[c] = T(0);
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
-            A[(NC + 9 + d) * 64 + l] = d < DIM ? st : T(0);
-        }
-      else
-        {
-#pragma unroll 1
-          for (int cb = 0; cb < NC; ++cb)
+          for (int d = 0; d < DIM; ++d)
             {
-              T v[NC], vr[NC];
+              T u[3] = {T(0), T(0), T(0)};
+              u[d]   = T(1);
+              if (cb <= 1)
+                tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, cb == 0 ? u : zero,
+                                             cb == 1 ? u : zero, gm[d + 1]);
+              else
 #pragma unroll
-              for (int c = 0; c < NC; ++c)
-                v[c] = c == cb ? T(1) : T(0);
-              tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
-              T x = T(0);
-#pragma unroll
-              for (int c = 0; c < NC; ++c)
-                if (c == cb)
-                  x = vr[c];
-              A[cb * 64 + l] = x;
+                for (int e = 0; e < DIM; ++e)
+                  gm[d + 1][e] = T(0);
             }
-          c_rows(0);
 #pragma unroll
           for (int d = 0; d < 3; ++d)
-            A[(NC + 9 + d) * 64 + l] = d < DIM ? T(1) : T(0);
-        }
-    }
-    __syncthreads();
-
-    // 2) q -> node sums, one pass per direction over all rows
-    //    (A -> B -> A in 3D, the last pass into registers)
-    T res[NR];
-    {
-      T K[3][3]; // [type][q] for the current direction
-      auto coefs = [&](int d) {
-        T sc[3], ec[3];
-        q2_s_col(L.t[d], sc);
-        q2_e_col(L.t[d], ec);
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
+            {
+              A[(NC + d) * 64 + l]     = d < DIM ? gm[0][d] : T(0);
+              A[(NC + 3 + d) * 64 + l] = d < DIM ? gm[d + 1][d] : T(0);
+            }
+          // pairs (0,1), (0,2), (1,2)
+          A[(NC + 6) * 64 + l] = gm[2][0] + gm[1][1];
+          A[(NC + 7) * 64 + l] = DIM == 3 ? gm[3 % (DIM + 1)][0] + gm[1][2 % DIM] : T(0);
+          A[(NC + 8) * 64 + l] = DIM == 3 ? gm[3 % (DIM + 1)][1] + gm[2][2 % DIM] : T(0);
+        };
+        if (b >= 0)
           {
-            const T w = q2_w<T>(q);
-            const T e = ec[q] * T(kp.inv_h[d]);
-            K[0][q]   = w * sc[q] * sc[q];
-            K[1][q]   = w * sc[q] * e;
-            K[2][q]   = w * e * e;
+            T v[NC], vr[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              v[c] = c == b ? T(1) : T(0);
+            tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              A[c * 64 + l] = vr[c];
+            c_rows(b);
+            const T st = b == 0 ? kc : (b >= 2 ? lkp : T(0));
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              A[(NC + 9 + d) * 64 + l] = d < DIM ? st : T(0);
           }
-      };
-      coefs(0);
+        else
+          {
+#pragma unroll 1
+            for (int cb = 0; cb < NC; ++cb)
+              {
+                T v[NC], vr[NC];
 #pragma unroll
-      for (int r = 0; r < NR; ++r)
-        B[r * 64 + l] = q2_pass(A + r * 64, L.base(0), 1, K[RW::type(NC, r, 0)]);
+                for (int c = 0; c < NC; ++c)
+                  v[c] = c == cb ? T(1) : T(0);
+                tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
+                T x = T(0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                  if (c == cb)
+                    x = vr[c];
+                A[cb * 64 + l] = x;
+              }
+            c_rows(0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              A[(NC + 9 + d) * 64 + l] = d < DIM ? T(1) : T(0);
+          }
+      }
       __syncthreads();
-      coefs(1);
-      if constexpr (DIM == 3)
-        {
+
+      // 2) q -> node sums, one pass per direction over all rows
+      //    (A -> B -> A in 3D, the last pass into registers)
+      T res[NR];
+      {
+        T K[3][3]; // [type][q] for the current direction
+        auto coefs = [&](int d) {
+          T sc[3], ec[3];
+          q2_s_col(L.t[d], sc);
+          q2_e_col(L.t[d], ec);
 #pragma unroll
-          for (int r = 0; r < NR; ++r)
-            A[r * 64 + l] = q2_pass(B + r * 64, L.base(1), 3, K[RW::type(NC, r, 1)]);
-          __syncthreads();
-          coefs(2);
-#pragma unroll
-          for (int r = 0; r < NR; ++r)
-            res[r] = q2_pass(A + r * 64, L.base(2), 9, K[RW::type(NC, r, 2)]);
-        }
-      else
+          for (int q = 0; q < 3; ++q)
+            {
+              const T w = q2_w<T>(q);
+              const T e = ec[q] * T(kp.inv_h[d]);
+              K[0][q]   = w * sc[q] * sc[q];
+              K[1][q]   = w * sc[q] * e;
+              K[2][q]   = w * e * e;
+            }
+        };
+        coefs(0);
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-          res[r] = q2_pass(B + r * 64, L.base(1), 3, K[RW::type(NC, r, 1)]);
-    }
+          B[r * 64 + l] = q2_pass(A + r * 64, L.base(0), 1, K[RW::type(NC, r, 0)]);
+        __syncthreads();
+        coefs(1);
+        if constexpr (DIM == 3)
+          {
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+              A[r * 64 + l] = q2_pass(B + r * 64, L.base(1), 3, K[RW::type(NC, r, 1)]);
+            __syncthreads();
+            coefs(2);
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+              res[r] = q2_pass(A + r * 64, L.base(2), 9, K[RW::type(NC, r, 2)]);
+          }
+        else
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+            res[r] = q2_pass(B + r * 64, L.base(1), 3, K[RW::type(NC, r, 1)]);
+      }
 
-    // 3) rows to outputs
-    T acc[NC];
+      // 3) rows to outputs
+      T acc[NC];
 #pragma unroll
-    for (int a = 0; a < NC; ++a)
-      acc[a] = res[a];
-    T c_sum = T(0), st_sum = T(0);
+      for (int a = 0; a < NC; ++a)
+        acc[a] = res[a];
+      T c_sum = T(0), st_sum = T(0);
 #pragma unroll
-    for (int r = NC; r < NC + 9; ++r)
-      c_sum += res[r];
+      for (int r = NC; r < NC + 9; ++r)
+        c_sum += res[r];
 #pragma unroll
-    for (int r = NC + 9; r < NR; ++r)
-      st_sum += res[r];
-    const T hp = T(kp.h_prod);
-    if (b >= 0)
+      for (int r = NC + 9; r < NR; ++r)
+        st_sum += res[r];
+      const T hp = T(kp.h_prod);
+      if (b >= 0)
+        {
+          acc[0] += c_sum;
+#pragma unroll
+          for (int a = 1; a < NC; ++a)
+            if (a == (b == 0 ? 1 : b) && b != 1)
+              acc[a] += st_sum;
+        }
+      else
+        {
+          acc[0] += c_sum;
+#pragma unroll
+          for (int a = 2; a < NC; ++a)
+            acc[a] += lkp * st_sum;
+        }
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < NC; ++a)
+        A[a * 64 + l] = hp * acc[a];
+      __syncthreads();
+    };
+    // column -2: all block columns, column b of a node's block at
+    // dst + node * os + b * NC (the workgroup's cache stays in the caches
+    // between columns); otherwise the one column (or the diagonal) asked for
+    const bool all = column < -1;
+    const int  b0 = all ? 0 : column, b1 = all ? NC : column + 1;
+#pragma unroll 1
+    for (int b = b0; b < b1; ++b)
       {
-        acc[0] += c_sum;
-#pragma unroll
-        for (int a = 1; a < NC; ++a)
-          if (a == (b == 0 ? 1 : b) && b != 1)
-            acc[a] += st_sum;
+        column_entries(b);
+        q2_scatter<DIM, NC, T>(kp, L, A, dst, os, all ? 1 : ocs, all ? b * NC : oo);
+        __syncthreads();
       }
-    else
-      {
-        acc[0] += c_sum;
-#pragma unroll
-        for (int a = 2; a < NC; ++a)
-          acc[a] += lkp * st_sum;
-      }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < NC; ++a)
-      A[a * 64 + l] = hp * acc[a];
-    __syncthreads();
-    q2_scatter<DIM, NC, T>(kp, L, A, dst, os, ocs, oo);
   }
 } // namespace hps
