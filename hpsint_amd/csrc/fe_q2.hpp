@@ -782,71 +782,155 @@ namespace hps
             float *__restrict__ cache32)
   {
     using CL         = CacheLayout<DIM, G, TENS>;
+    using LN         = Q2Lane<DIM>;
     constexpr int NC = G + 2;
     constexpr int NF = CL::n_fields;
+    constexpr int NP = DIM * (DIM + 1) / 2;
+    constexpr int LP = LN::LP;
     using T          = double;
 
     __shared__ T U[NC * 64];
-    __shared__ T W[3][64];
+    __shared__ T A[NC * 64];
+    __shared__ T LG[TENS ? G * DIM * 64 : 1]; // grad eta_i, this lane's entries
 
-    const Q2Lane<DIM> L(kp);
+    const LN  L(kp);
+    const int l = L.l;
     q2_gather<DIM, NC, T>(kp, L, U, [&](int64_t n, T *x) { q2_load_row<NC>(lin + n * NC, x); });
     __syncthreads();
-
     const T inv_h[3] = {kp.inv_h[0], kp.inv_h[1], kp.inv_h[2]};
-    T       srow[DIM][3], drow[DIM][3];
+
+    // values at the Gauss points, one pass per direction for all components
+    T v[NC];
+    {
+      T srow[DIM][3];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        q2_s_row(L.t[d], srow[d]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        A[c * 64 + l] = q2_pass(U + c * 64, L.base(0), 1, srow[0]);
+      __syncthreads();
+      if constexpr (DIM == 3)
+        {
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            U[c * 64 + l] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
+          __syncthreads();
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            v[c] = q2_pass(U + c * 64, L.base(2), 9, srow[2]);
+        }
+      else
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          v[c] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
+    }
+    T *QV = DIM == 3 ? A : U; // the buffer the last pass did not read
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      QV[c * 64 + l] = v[c];
+    __syncthreads();
+
+    // cache: [workgroup][field][LP lanes]; lanes past the cell slots hold
+    // nothing
+    const bool    wr = l < LP;
+    const int64_t o  = ((kp.wg_base + blockIdx.x) * NF) * LP + l;
+    auto          put = [&](int f, T x) {
+      if (wr)
+        {
+          cache64[o + f * LP] = x;
+          if (cache32)
+            cache32[o + f * LP] = (float)x;
+        }
+    };
+    put(CL::c, v[0]);
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+      put(CL::eta + i, v[2 + i]);
+    T drow[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      q2_dq_row(L.t[d], drow[d]);
+    T lmu[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       {
-        q2_s_row(L.t[d], srow[d]);
-        q2_dq_row(L.t[d], drow[d]);
-      }
-    T f[NF];
-    T leg[TENS ? G : 1][DIM]; // grad eta_i (tensorial mobility)
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-      {
-        T v, g[DIM];
-        q2_interp_vg<DIM>(L, U + c * 64, W[0], W[1], W[2], srow, drow, inv_h, v, g);
-        if (c == 0)
-          {
-            f[CL::c] = v;
-            if constexpr (TENS)
-#pragma unroll
-              for (int d = 0; d < DIM; ++d)
-                f[CL::grad_c + d] = g[d];
-          }
-        else if (c == 1)
-          {
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              f[CL::grad_mu + d] = g[d];
-          }
-        else
-          {
-            f[CL::eta + c - 2] = v;
-            if constexpr (TENS)
-#pragma unroll
-              for (int d = 0; d < DIM; ++d)
-                leg[c - 2][d] = g[d];
-          }
+        lmu[d] = q2_pass(QV + 64, L.base(d), LN::str(d), drow[d]) * inv_h[d];
+        put(CL::grad_mu + d, lmu[d]);
       }
     if constexpr (TENS)
-      gb_fields<DIM, G>(f + CL::eta, leg, f + CL::grad_mu, f[CL::gb_s], f + CL::gb_p,
-                        f + CL::gb_q);
-    // cache: [workgroup][field][LP lanes]; lanes past the cell slots hold
-    // nothing
-    constexpr int LP = Q2Lane<DIM>::LP;
-    if (L.l < LP)
       {
-        const int64_t o = ((kp.wg_base + blockIdx.x) * NF) * LP + L.l;
 #pragma unroll
-        for (int q = 0; q < NF; ++q)
+        for (int d = 0; d < DIM; ++d)
+          put(CL::grad_c + d, q2_pass(QV, L.base(d), LN::str(d), drow[d]) * inv_h[d]);
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            LG[(i * DIM + d) * 64 + l] =
+              q2_pass(QV + (2 + i) * 64, L.base(d), LN::str(d), drow[d]) * inv_h[d];
+        // the grain-boundary sums of gb_fields (physics.hpp), same pair
+        // order and arithmetic; grad eta_i and eta_i are this lane's LDS
+        // entries, the pair loop runs over i with j unrolled (Q in registers)
+        T S = T(0), P[NP], Q[G * DIM];
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+          P[k] = T(0);
+#pragma unroll
+        for (int k = 0; k < G * DIM; ++k)
+          Q[k] = T(0);
+#pragma unroll 1
+        for (int i = 1; i < G; ++i)
           {
-            cache64[o + q * LP] = f[q];
-            if (cache32)
-              cache32[o + q * LP] = (float)f[q];
+            T li[DIM], qi[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              {
+                li[d] = LG[(i * DIM + d) * 64 + l];
+                qi[d] = T(0);
+              }
+            const T ei = QV[(2 + i) * 64 + l];
+#pragma unroll
+            for (int j = 0; j < G - 1; ++j)
+              if (j < i)
+                {
+                  T egd[DIM];
+#pragma unroll
+                  for (int d = 0; d < DIM; ++d)
+                    egd[d] = li[d] - LG[(j * DIM + d) * 64 + l];
+                  const T filter = unit_vector_filter_2<DIM>(egd);
+                  const T ej     = QV[(2 + j) * 64 + l];
+                  const T ee     = ei * ej;
+                  S += ee;
+#pragma unroll
+                  for (int a = 0; a < DIM; ++a)
+#pragma unroll
+                    for (int b = a; b < DIM; ++b)
+                      P[sym_index<DIM>(a, b)] += ee * filter * egd[a] * egd[b];
+                  const T elmu = vdot<DIM>(egd, lmu);
+#pragma unroll
+                  for (int d = 0; d < DIM; ++d)
+                    {
+                      const T r = (filter * egd[d]) * elmu;
+                      qi[d] += ej * r;
+                      Q[j * DIM + d] += ei * r;
+                    }
+                }
+            // row i's own pairs come before its pairs as the second grain
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+              if (k == i)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  Q[k * DIM + d] += qi[d];
           }
+        put(CL::gb_s, S);
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+          put(CL::gb_p + k, P[k]);
+#pragma unroll
+        for (int k = 0; k < G * DIM; ++k)
+          put(CL::gb_q + k, Q[k]);
       }
   }
 
