@@ -35,7 +35,7 @@ VMULT_TRAFFIC_128 = 1.3841e9
 # FP64 flop per launch of the residual main kernel on the same workload
 # (SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes, FMA = 2;
 # profiles/q1_kernels_128cubed_fp64_counters.md); FP64 vector peak
-RESIDUAL_FLOP_128 = 8.35e9
+RESIDUAL_FLOP_128 = 8.25e9
 FP64_PEAK_TFLOPS = 78.6
 
 
