@@ -143,6 +143,7 @@ class SinteringOperatorGeneric:
         free_energy = free_energy or FreeEnergyParams()
         mobility = mobility or MobilityParams()
         self.mesh = mesh
+        self.fe_degree = fe_degree
         self.device = torch.device(device)
         p = _ffi.HpsParams()
         p.dim = mesh.dim

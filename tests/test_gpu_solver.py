@@ -111,3 +111,33 @@ def test_q2_newton_step_satisfies_oracle_residual(tensorial, inner_fp32):
     r0 = np.linalg.norm(po.residual(p, 2, u0, [u0]))
     r = np.linalg.norm(po.residual(p, 2, xs, [u0]))
     assert r <= 2e-8 * r0, (r, r0)
+
+
+def test_restart_continues_the_time_loop(tmp_path):
+    """BDF2 two-particle run: 2 steps, restart file, 2 more steps in a new
+    stepper (S/driver.h:2535-2610) against 4 uninterrupted steps"""
+    from hpsint_amd.restart import load_restart, save_restart
+    op, mesh, u0 = setup_two_particles()
+
+    def stepper():
+        s = TimeStepper(op, NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2), order=2,
+                        dt=1e-3)
+        s.initialize(torch.from_numpy(u0).cuda())
+        return s
+
+    ref = stepper()
+    for _ in range(4):
+        ref.make_step()
+    a = stepper()
+    for _ in range(2):
+        a.make_step()
+    path = str(tmp_path / "restart.safetensors")
+    save_restart(path, a)
+    b = stepper()
+    assert load_restart(path, b) == 2
+    for _ in range(2):
+        b.make_step()
+    assert abs(b.t - ref.t) < 1e-15
+    x, y = b.u.cpu().numpy(), ref.u.cpu().numpy()
+    # same equations and history; only the solver's reuse state differs
+    assert np.linalg.norm(x - y) <= 1e-6 * np.linalg.norm(y)

@@ -1,0 +1,68 @@
+"""Restart files and VTK output (hpsint_amd/restart.py) on the CPU: the
+restart round trip restores the time-integrator state exactly and refuses a
+file written for another mesh; the VTK file reads back bit-exactly with the
+mesh geometry and the indicator fields bnds and gb of the reference's
+output (S/operator_sintering_base.h:171-183)."""
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from hpsint_amd import CartesianMesh, TimeIntegratorData
+from hpsint_amd.restart import load_restart, read_vtk, save_restart, write_vtk
+
+
+def fake_stepper(mesh, nc, seed, order=2, device="cpu"):
+    rng = np.random.default_rng(seed)
+    nn = int(np.prod([n + 1 for n in mesh.n_cells]))
+    op = types.SimpleNamespace(mesh=mesh, fe_degree=1, device=torch.device(device),
+                               n_components=lambda: nc)
+    st = types.SimpleNamespace(op=op, order=order, t=0.0, dt=1e-3, records=[None] * 7,
+                               td=TimeIntegratorData(order=order, dt=1e-3))
+    st.u = torch.from_numpy(rng.standard_normal(nn * nc))
+    st.history = [torch.from_numpy(rng.standard_normal(nn * nc)) for _ in range(order)]
+    st.td.update_dt(2e-3)
+    st.td.update_dt(1.5e-3)
+    st.t = 0.0123456789
+    st.dt = 7.5e-4
+    return st
+
+
+def test_restart_round_trip(tmp_path):
+    mesh = CartesianMesh((6, 5, 4), (0.0, 0.0, 0.0), (6.0, 5.0, 4.0))
+    a = fake_stepper(mesh, 6, seed=1)
+    path = str(tmp_path / "r.safetensors")
+    save_restart(path, a)
+    b = fake_stepper(mesh, 6, seed=2)
+    n = load_restart(path, b)
+    assert n == 7
+    assert torch.equal(a.u, b.u)
+    assert len(b.history) == 2 and all(torch.equal(x, y) for x, y in zip(a.history, b.history))
+    assert a.td.dt == b.td.dt and a.td.weights == b.td.weights
+    assert a.t == b.t and a.dt == b.dt
+    # another mesh is refused
+    c = fake_stepper(CartesianMesh((6, 5, 5), (0.0, 0.0, 0.0), (6.0, 5.0, 5.0)), 6, seed=3)
+    with pytest.raises(ValueError):
+        load_restart(path, c)
+
+
+@pytest.mark.parametrize("cells", [(7, 3), (4, 3, 2)])
+def test_vtk_reads_back(tmp_path, cells):
+    dim = len(cells)
+    mesh = CartesianMesh(cells, (-1.0,) * dim, tuple(0.5 * n - 1.0 for n in cells))
+    nc = 5
+    nn = int(np.prod([n + 1 for n in cells]))
+    u = np.random.default_rng(4).uniform(0, 1, size=(nn, nc))
+    path = str(tmp_path / "s.vtk")
+    write_vtk(path, mesh, u.ravel(), nc)
+    dims, origin, spacing, f = read_vtk(path)
+    assert dims == [n + 1 for n in cells] + [1] * (3 - dim)
+    assert origin[:dim] == [-1.0] * dim and spacing[:dim] == [0.5] * dim
+    assert np.array_equal(f["c"], u[:, 0]) and np.array_equal(f["mu"], u[:, 1])
+    for i in range(nc - 2):
+        assert np.array_equal(f["eta%d" % i], u[:, 2 + i])
+    e = u[:, 2:]
+    np.testing.assert_allclose(f["bnds"], (e ** 2).sum(axis=1), rtol=1e-15)
+    gb = sum(e[:, i] * e[:, j] for i in range(nc - 2) for j in range(i))
+    np.testing.assert_allclose(f["gb"], gb, rtol=1e-14)
