@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--train-iterations", type=int, default=6000)
     ap.add_argument("--q2-cells", type=int, default=96,
                     help="cells per direction of the Q2 tensorial workload (0 = skip)")
+    ap.add_argument("--q2-timesteps", type=int, default=2,
+                    help="timed Newton steps per method on the Q2 workload (0: none)")
     ap.add_argument("--mp-cells", type=int, default=192,
                     help="cells per direction of the 27-sphere mixed-precision workload "
                          "(0 = skip)")
@@ -237,9 +239,47 @@ def q2_vmult(args, timed):
            "n_dofs": op.n_dofs, "vmult_f64_gdofs": op.n_dofs / t64 / 1e9,
            "vmult_f64_ms": t64 * 1e3, "vmult_f32_gdofs": op.n_dofs / t32 / 1e9,
            "vmult_f64_hbm_frac": bytes64 / t64 / 1e9 / HBM_PEAK_GBS}
-    del op, u, src, dst, src32, dst32
+    del src, dst, src32, dst32
+    if args.q2_timesteps > 0:
+        out["timesteps"] = q2_timesteps(op, u, args)
+    del op, u
     torch.cuda.empty_cache()
     return out
+
+
+def q2_timesteps(op, u0, args):
+    """s/timestep on the Q2 workload (BDF1, dt = 1e-3, Newton rtol 1e-8, GMRES
+    rtol 1e-2, node-block Jacobi): plain FP64 and FP32 inner Krylov (no
+    predictor), after one untimed step from the initial fields"""
+    import torch
+    from hpsint_amd.solvers import NewtonSolver, TimeStepper
+    dt = 1e-3
+
+    def stepper(fp32):
+        solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
+                              preconditioner="InverseComponentBlockDiagonalMatrix",
+                              inner_fp32=fp32)
+        return TimeStepper(op, solver, order=1, dt=dt)
+
+    first = stepper(False)
+    first.initialize(u0)
+    first.make_step()
+    u1 = first.u.clone()
+    del first
+    res = {}
+    for name, fp32 in (("fp64_plain", False), ("mp_only", True)):
+        s = stepper(fp32)
+        s.initialize(u1)
+        s.history = [u1.clone(), u0.clone()]
+        s.t = dt
+        rs = [s.make_step() for _ in range(args.q2_timesteps)]
+        res[name] = {"s_per_step": sum(r.seconds for r in rs) / len(rs),
+                     "newton": [r.newton_iterations for r in rs],
+                     "linear": [r.linear_iterations for r in rs]}
+        del s
+        torch.cuda.empty_cache()
+    res["speedup_mp_vs_fp64"] = res["fp64_plain"]["s_per_step"] / res["mp_only"]["s_per_step"]
+    return res
 
 
 def timestep_comparison(op, u0, dt, args):
