@@ -551,7 +551,6 @@ static int precond_update(hps_operator op, int kind, bool fp32)
     return fail(HPS_ERR_NOT_INITIALIZED, "preconditioner update before hps_set_linearization");
   const int64_t n = op->n_dofs, nn = op->n_nodes;
   const int     nc = op->nc;
-  fp32 = fp32 && !op->q2; // the Q2 kernels assemble in FP64
   if (kind == HPS_PRECON_POINT_JACOBI)
     {
       if (!op->diag)
@@ -560,7 +559,9 @@ static int precond_update(hps_operator op, int kind, bool fp32)
           HPS_HIP(hipMalloc(&op->inv_diag64, n * sizeof(double)));
           HPS_HIP(hipMalloc(&op->inv_diag32, n * sizeof(float)));
         }
-      if (op->q2)
+      if (op->q2 && fp32)
+        op->q2->probe32(q2_params(op), op->cache32, op->inv_diag32, -1, nc, 1, 0, op->stream);
+      else if (op->q2)
         op->q2->probe(q2_params(op), op->cache64, op->diag, -1, nc, 1, 0, op->stream);
       else if (fp32)
         op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_diag32, (float *)op->part,
@@ -595,7 +596,12 @@ static int precond_update(hps_operator op, int kind, bool fp32)
       // (blocks[node][column][row]): a block column is a contiguous run
       if (op->q2) // all block columns in one pass over the cache
         {
-          op->q2->probe(q2_params(op), op->cache64, op->blocks, -2, nc * nc, 1, 0, op->stream);
+          if (fp32)
+            op->q2->probe32(q2_params(op), op->cache32, op->inv_blocks32, -2, nc * nc, 1, 0,
+                            op->stream);
+          else
+            op->q2->probe(q2_params(op), op->cache64, op->blocks, -2, nc * nc, 1, 0,
+                          op->stream);
           HPS_LAUNCHED();
         }
       for (int j = 0; j < nc && !op->q2; ++j)

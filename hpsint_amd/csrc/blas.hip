@@ -382,11 +382,13 @@ namespace hps
               }
           }
       }
+    // inverses entry-major ([row * NC + column][node]): the apply reads
+    // each entry for consecutive nodes in one coalesced access
     for (int i = 0; i < NC; ++i)
       for (int j = 0; j < NC; ++j)
         {
-          inv64[(node * NC + i) * NC + j] = b[i][j];
-          inv32[(node * NC + i) * NC + j] = (float)b[i][j];
+          inv64[(int64_t)(i * NC + j) * n_nodes + node] = b[i][j];
+          inv32[(int64_t)(i * NC + j) * n_nodes + node] = (float)b[i][j];
         }
   }
 
@@ -472,8 +474,8 @@ namespace hps
 #pragma unroll
       for (int k = 0; k < NC; ++k)
         {
-          inv64[(node * NC + col) * NC + k] = b[k];
-          inv32[(node * NC + col) * NC + k] = (float)b[k];
+          inv64[(int64_t)(col * NC + k) * n_nodes + node] = b[k];
+          inv32[(int64_t)(col * NC + k) * n_nodes + node] = (float)b[k];
         }
   }
 
@@ -494,7 +496,7 @@ namespace hps
             T acc = T(0);
 #pragma unroll
             for (int j = 0; j < NC; ++j)
-              acc += inv[(node * NC + i) * NC + j] * s[j];
+              acc += inv[(int64_t)(i * NC + j) * n_nodes + node] * s[j];
             dst[node * NC + i] = acc;
           }
       }

@@ -985,17 +985,16 @@ namespace hps
     }
   };
 
-  template <int DIM, int G, bool TENS>
+  template <int DIM, int G, bool TENS, typename T>
   __global__ void __launch_bounds__(64)
-    q2_probe(const Q2Params kp, const double *__restrict__ cache, double *__restrict__ dst,
-             int column, int os, int ocs, int oo)
+    q2_probe(const Q2Params kp, const T *__restrict__ cache, T *__restrict__ dst, int column,
+             int os, int ocs, int oo)
   {
     using CL         = CacheLayout<DIM, G, TENS>;
     constexpr int NC = G + 2;
     constexpr int NF = CL::n_fields;
     constexpr int NR = NC + 12;
     using LN         = Q2Lane<DIM>;
-    using T          = double;
     using RW         = Q2ProbeRows<DIM>;
 
     __shared__ T A[NR * 64];

@@ -100,6 +100,8 @@ namespace hps
     void (*fill)(const Q2Params &, const double *lin, double *c64, float *c32, hipStream_t);
     void (*probe)(const Q2Params &, const double *cache, double *dst, int column, int ostride,
                   int ocstride, int ooffset, hipStream_t);
+    void (*probe32)(const Q2Params &, const float *cache, float *dst, int column, int ostride,
+                    int ocstride, int ooffset, hipStream_t);
   };
 
   // parameters of colour `col` (bit d = parity along d); wg: running count
@@ -183,13 +185,20 @@ namespace hps
                       int ocs, int oo, hipStream_t s)
     {
       colours(kp, [&](const Q2Params &k, unsigned nb) {
-        q2_probe<DIM, G, TENS><<<nb, 64, 0, s>>>(k, cache, dst, column, os, ocs, oo);
+        q2_probe<DIM, G, TENS, double><<<nb, 64, 0, s>>>(k, cache, dst, column, os, ocs, oo);
+      });
+    }
+    static void probe32(const Q2Params &kp, const float *cache, float *dst, int column, int os,
+                        int ocs, int oo, hipStream_t s)
+    {
+      colours(kp, [&](const Q2Params &k, unsigned nb) {
+        q2_probe<DIM, G, TENS, float><<<nb, 64, 0, s>>>(k, cache, dst, column, os, ocs, oo);
       });
     }
     static Q2Table table()
     {
       return Q2Table{CacheLayout<DIM, G, TENS>::n_fields, CPW, &vmult64, &vmult32,
-                     &residual, &fill, &probe};
+                     &residual, &fill, &probe, &probe32};
     }
   };
 

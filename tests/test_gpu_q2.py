@@ -140,3 +140,29 @@ def test_q2_diagonal_and_node_blocks_match_oracle(dim, cells, G, tens):
     P.vmult(y, dev(x))
     yr = np.linalg.solve(blocks_ref, x.reshape(-1, p.n_comp)[..., None])[..., 0].ravel()
     assert rel_l2(host(y), yr) <= 1e-11
+
+
+@pytest.mark.parametrize("dim,cells,G,tens", [
+    (3, (5, 4, 3), 2, False), (3, (3, 2, 3), 3, True), (2, (9, 6), 2, False)])
+def test_q2_fp32_preconditioner_assembly(dim, cells, G, tens):
+    """diagonal and node blocks assembled from the FP32 cache in FP32:
+    relative l2 error against the FP64 oracle <= 1e-5"""
+    import ctypes
+    from hpsint_amd import _ffi
+    op, p = make_pair(dim, cells, G, tens, w0=1000.0)
+    u = fields(p, 6)
+    blocks_ref = po.node_blocks(p, po.fill_qp(p, u))
+    op.fill_quadrature_point_values(dev(u))
+    P = Preconditioner(op, "InverseComponentBlockDiagonalMatrix")
+    P.do_update(fp32=True)
+    nc = p.n_comp
+    blocks = torch.empty(op.n_nodes * nc * nc, dtype=torch.float64, device="cuda")
+    _ffi.check(_ffi.lib().hps_precond_get_blocks(op.handle, ctypes.c_void_p(blocks.data_ptr())))
+    b = host(blocks).reshape(blocks_ref.shape)
+    assert rel_l2(b, blocks_ref) <= 1e-5, rel_l2(b, blocks_ref)
+    D = Preconditioner(op, "InverseDiagonalMatrix")
+    D.do_update(fp32=True)
+    inv = op.initialize_dof_vector()
+    d = np.einsum("nii->ni", blocks_ref).ravel()
+    _ffi.check(_ffi.lib().hps_precond_get_diagonal(op.handle, ctypes.c_void_p(inv.data_ptr())))
+    assert rel_l2(host(inv), np.where(np.abs(d) > 1e-10, 1.0 / d, 1.0)) <= 1e-5
