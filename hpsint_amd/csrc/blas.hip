@@ -398,16 +398,18 @@ namespace hps
   // magnitude among the rows not yet used (lowest row on ties); pivot rows
   // are not moved, so the lane that pivots column j ends with row j of the
   // inverse.  The normalised pivot row reaches the group through LDS.
+  // (16 groups per workgroup: the entry-major writes of the inverses cover
+  // 16 consecutive nodes)
   template <int NC>
-  __global__ void __launch_bounds__(64)
+  __global__ void __launch_bounds__(256)
     k_invert_blocks_rows(int64_t n_nodes, const double *__restrict__ blocks,
                          double *__restrict__ inv64, float *__restrict__ inv32,
                          int *__restrict__ singular)
   {
     static_assert(NC <= 16, "one row per lane of a 16-lane group");
-    __shared__ double piv[4][2 * NC];
+    __shared__ double piv[16][2 * NC];
     const int     grp  = threadIdx.x >> 4, i = threadIdx.x & 15;
-    const int64_t node = (int64_t)blockIdx.x * 4 + grp;
+    const int64_t node = (int64_t)blockIdx.x * 16 + grp;
     const bool    live = node < n_nodes && i < NC;
     double        a[NC], b[NC];
 #pragma unroll
@@ -682,12 +684,12 @@ namespace hps
   {
     // up to 6 components both matrices stay in one lane's registers
     const int g  = (int)((n_nodes + 63) / 64);
-    const int g4 = (int)((n_nodes + 3) / 4);
+    const int g16 = (int)((n_nodes + 15) / 16);
 #define HPS_CALL(N)                                                                          \
   if constexpr (N <= 6)                                                                      \
     k_invert_blocks<N><<<g, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular);            \
   else                                                                                       \
-    k_invert_blocks_rows<N><<<g4, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
+    k_invert_blocks_rows<N><<<g16, 256, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
     return true;

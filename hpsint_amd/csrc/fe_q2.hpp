@@ -631,6 +631,172 @@ namespace hps
       }
   }
 
+  // residual of the workgroup's cells, with the passes of all components
+  // between two barriers as in q2_vmult_body.  U holds the gathered
+  // iterate (in: nodal, out: integrated rows), H the history combination
+  // sum_k w_k old_k (OP_RES2), A is the second work buffer.
+  template <int DIM, int G, bool TENS, int OP, typename T>
+  __device__ __forceinline__ void q2_residual_body(const Q2Params &kp, const Q2Lane<DIM> &L,
+                                                   T *U, T *H, T *A, const T *inv_h)
+  {
+    using LN           = Q2Lane<DIM>;
+    constexpr int  NC  = G + 2;
+    constexpr bool HIST = OP == OP_RES2;
+    const int      l   = L.l;
+    T              srow[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      q2_s_row(L.t[d], srow[d]);
+    // values of the iterate (U -> A -> U in 3D, U -> A in 2D)
+    T v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      A[c * 64 + l] = q2_pass(U + c * 64, L.base(0), 1, srow[0]);
+    __syncthreads();
+    if constexpr (DIM == 3)
+      {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          U[c * 64 + l] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          v[c] = q2_pass(U + c * 64, L.base(2), 9, srow[2]);
+      }
+    else
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        v[c] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
+    // values of the history combination (H -> A -> H in 3D, H -> U in 2D;
+    // every buffer written here was last read before the previous barrier)
+    T hist[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      hist[c] = T(0);
+    if constexpr (HIST)
+      {
+        if constexpr (DIM == 3)
+          {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              A[c * 64 + l] = q2_pass(H + c * 64, L.base(0), 1, srow[0]);
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              H[c * 64 + l] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if (c != 1)
+                hist[c] = q2_pass(H + c * 64, L.base(2), 9, srow[2]);
+          }
+        else
+          {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              U[c * 64 + l] = q2_pass(H + c * 64, L.base(0), 1, srow[0]);
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if (c != 1)
+                hist[c] = q2_pass(U + c * 64, L.base(1), 3, srow[1]);
+          }
+      }
+    __syncthreads();
+    // gradients from the Gauss-point values in A
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      A[c * 64 + l] = v[c];
+    __syncthreads();
+    T g[NC][DIM];
+    {
+      T drow[DIM][3];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        q2_dq_row(L.t[d], drow[d]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+          g[c][d] = q2_pass(A + c * 64, L.base(d), LN::str(d), drow[d]) * inv_h[d];
+    }
+    T vr[NC], gr[NC][DIM];
+    residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, v, g, hist, vr, gr);
+    T jxw = T(kp.h_prod);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      jxw *= q2_w<T>(L.t[d]);
+    __syncthreads();
+    // integration: H takes the value rows, U and A the derivative passes
+    T scol[DIM][3], dcol[DIM][3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        q2_s_col(L.t[d], scol[d]);
+        q2_dq_col(L.t[d], dcol[d]);
+      }
+    T y[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      {
+        y[c]           = vr[c] * jxw;
+        U[c * 64 + l]  = gr[c][0] * (jxw * inv_h[0]);
+        A[c * 64 + l]  = gr[c][1] * (jxw * inv_h[1]);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      y[c] += q2_pass(U + c * 64, L.base(0), 1, dcol[0]) +
+              q2_pass(A + c * 64, L.base(1), 3, dcol[1]);
+    __syncthreads();
+    if constexpr (DIM == 3)
+      {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          U[c * 64 + l] = gr[c][2] * (jxw * inv_h[2]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] += q2_pass(U + c * 64, L.base(2), 9, dcol[2]);
+        __syncthreads();
+        // z, y, x: A -> U -> A -> U
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          A[c * 64 + l] = y[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          U[c * 64 + l] = q2_pass(A + c * 64, L.base(2), 9, scol[2]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          A[c * 64 + l] = q2_pass(U + c * 64, L.base(1), 3, scol[1]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] = q2_pass(A + c * 64, L.base(0), 1, scol[0]);
+      }
+    else
+      {
+        // y, x: A -> U -> registers
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          A[c * 64 + l] = y[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          U[c * 64 + l] = q2_pass(A + c * 64, L.base(1), 3, scol[1]);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          y[c] = q2_pass(U + c * 64, L.base(0), 1, scol[0]);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      U[c * 64 + l] = y[c];
+  }
+
   // ----------------------------------------- tangent vmult and residual
   template <int DIM, int G, bool TENS, int OP, typename T>
   __global__ void __launch_bounds__(64)
@@ -645,8 +811,7 @@ namespace hps
 
     __shared__ T U[NC * 64];
     __shared__ T H[(HIST || OP == OP_VMULT) ? NC * 64 : 1];
-    __shared__ T W[OP == OP_VMULT ? 1 : 4][64];
-    __shared__ T A[OP == OP_VMULT ? NC * 64 : 1];
+    __shared__ T A[NC * 64];
 
     const Q2Lane<DIM> L(kp);
     q2_gather<DIM, NC, T>(kp, L, U, [&](int64_t n, T *x) { q2_load_row<NC>(src + n * NC, x); });
@@ -677,100 +842,9 @@ namespace hps
 
     const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
     if constexpr (OP == OP_VMULT)
-      {
-        q2_vmult_body<DIM, G, TENS>(kp, L, U, H, A, cache, inv_h);
-        __syncthreads();
-        q2_scatter<DIM, NC, T>(kp, L, U, dst, NC, 1, 0);
-        return;
-      }
-    T v[NC], g[NC][DIM];
-    {
-      T srow[DIM][3], drow[DIM][3];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d)
-        {
-          q2_s_row(L.t[d], srow[d]);
-          q2_dq_row(L.t[d], drow[d]);
-        }
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-        q2_interp_vg<DIM>(L, U + c * 64, W[0], W[1], W[2], srow, drow, inv_h, v[c], g[c]);
-    }
-
-    // tangent: the value rows here; the gradient rows are formed one
-    // component at a time right before that component's integration (each
-    // is a multiple of one input gradient, except the c row), which keeps
-    // fewer values live
-    T vr[NC], gr[NC][DIM];
-    T lin[OP == OP_VMULT ? NF : 1];
-    if constexpr (OP == OP_VMULT)
-      {
-        const T *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * Q2Lane<DIM>::LP + L.l;
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
-          lin[f] = L.valid ? cl[f * Q2Lane<DIM>::LP] : T(0);
-        tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
-      }
+      q2_vmult_body<DIM, G, TENS>(kp, L, U, H, A, cache, inv_h);
     else
-      {
-        T hist[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          hist[c] = T(0);
-        if constexpr (HIST)
-          {
-            T srow[DIM][3];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              q2_s_row(L.t[d], srow[d]);
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-              if (c != 1)
-                hist[c] = q2_interp_v<DIM>(L, H + c * 64, W[0], W[1], srow);
-          }
-        residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, v, g, hist, vr, gr);
-      }
-
-    T jxw = T(kp.h_prod);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      jxw *= q2_w<T>(L.t[d]);
-    {
-      T scol[DIM][3], dcol[DIM][3];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d)
-        {
-          q2_s_col(L.t[d], scol[d]);
-          q2_dq_col(L.t[d], dcol[d]);
-        }
-      // (eta rows first, the c row last: the inputs die in that order)
-#pragma unroll
-      for (int cc = 0; cc < NC; ++cc)
-        {
-          const int c = NC - 1 - cc;
-          T         gc[DIM];
-          if constexpr (OP == OP_VMULT)
-            {
-              if (c == 0)
-                tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g[0], g[1], gc);
-              else
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  gc[d] = c == 1 ? -T(kp.pp.kappa_c) * g[0][d]
-                                 : T(kp.pp.L) * T(kp.pp.kappa_p) * g[c][d];
-            }
-          else
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              gc[d] = gr[c][d];
-          T gq[DIM];
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            gq[d] = gc[d] * (jxw * inv_h[d]);
-          const T r = q2_integrate<DIM>(L, vr[c] * jxw, gq, W[0], W[1], W[2], W[3], scol, dcol);
-          U[c * 64 + L.l] = r;
-        }
-    }
+      q2_residual_body<DIM, G, TENS, OP>(kp, L, U, H, A, inv_h);
     __syncthreads();
     q2_scatter<DIM, NC, T>(kp, L, U, dst, NC, 1, 0);
   }
