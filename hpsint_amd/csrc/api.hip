@@ -85,6 +85,11 @@ struct hps_operator_s
   hipEvent_t           ev[3];
   bool                 timing;
   const double        *old[3];
+  double               cut_tol;          // grain cut-off tolerance (< 0: off)
+  bool                 cut_ready;        // tile groups of the current linearisation
+  unsigned            *cut_masks;        // per tile: bit g = eta_g active
+  int                 *cut_tiles;        // tiles grouped by active count
+  int                  cut_off[hps::CUT_MAX_GA + 2];
   int64_t              n_constrained;    // constrained nodes (none by default)
   int64_t             *constrained;      // their node indices (device)
   double              *constrained_save; // n_constrained * nc saved src entries
@@ -156,6 +161,7 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   hps_operator op = new hps_operator_s();
   std::memset((void *)op, 0, sizeof(*op));
   op->p            = *prm;
+  op->cut_tol      = -1.0;
   op->dim          = prm->dim;
   op->n_grains     = G;
   op->nc           = G + 2;
@@ -305,6 +311,8 @@ extern "C" int hps_destroy(hps_operator op)
   (void)hipFree(op->d_singular);
   (void)hipFree(op->constrained);
   (void)hipFree(op->constrained_save);
+  (void)hipFree(op->cut_masks);
+  (void)hipFree(op->cut_tiles);
   for (int i = 0; i < 3; ++i)
     if (op->ev[i])
       (void)hipEventDestroy(op->ev[i]);
@@ -354,6 +362,63 @@ extern "C" int hps_set_time_weights(hps_operator op, double w0, double w_alg,
   return HPS_OK;
 }
 
+// per-tile active order parameters of the linearisation point u, tiles
+// grouped by their number of active order parameters
+static int compute_cut(hps_operator op, const double *u)
+{
+  op->cut_ready = false;
+  const int nt  = op->n_tiles;
+  if (!op->cut_masks)
+    {
+      HPS_HIP(hipMalloc(&op->cut_masks, nt * sizeof(unsigned)));
+      HPS_HIP(hipMalloc(&op->cut_tiles, nt * sizeof(int)));
+    }
+  hps::tile_masks_d3(op->kp, nt, op->n_grains, op->cut_tol, u, op->cut_masks, op->stream);
+  HPS_LAUNCHED();
+  std::vector<unsigned> m(nt);
+  HPS_HIP(hipMemcpyAsync(m.data(), op->cut_masks, nt * sizeof(unsigned), hipMemcpyDeviceToHost,
+                         op->stream));
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  int count[hps::CUT_MAX_GA + 1] = {};
+  for (int i = 0; i < nt; ++i)
+    {
+      const int ga = __builtin_popcount(m[i]);
+      if (ga > hps::CUT_MAX_GA)
+        return HPS_OK; // a tile with more active order parameters: full vmult
+      ++count[ga];
+    }
+  op->cut_off[0] = 0;
+  for (int g = 0; g <= hps::CUT_MAX_GA; ++g)
+    op->cut_off[g + 1] = op->cut_off[g] + count[g];
+  std::vector<int> list(nt);
+  int              pos[hps::CUT_MAX_GA + 1];
+  for (int g = 0; g <= hps::CUT_MAX_GA; ++g)
+    pos[g] = op->cut_off[g];
+  for (int i = 0; i < nt; ++i)
+    list[pos[__builtin_popcount(m[i])]++] = i;
+  HPS_HIP(hipMemcpy(op->cut_tiles, list.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+  op->cut_ready = true;
+  return HPS_OK;
+}
+
+extern "C" int hps_set_grain_cut_off(hps_operator op, double tol)
+{
+  if (int r = check_op(op))
+    return r;
+  if (tol < 0.0)
+    {
+      op->cut_tol   = -1.0;
+      op->cut_ready = false;
+      return HPS_OK;
+    }
+  if (op->q2 || op->dim != 3 || op->tens || op->ops->ty != hps::TY3 || op->n_grains > 32)
+    return fail(HPS_ERR_INVALID_ARGUMENT,
+                "grain cut-off is built for Q1, 3D, scalar mobility, default tiles");
+  op->cut_tol   = tol;
+  op->cut_ready = false; // the tile groups follow at the next linearisation
+  return HPS_OK;
+}
+
 extern "C" int hps_set_linearization(hps_operator op, const double *u)
 {
   if (int r = check_op(op))
@@ -371,6 +436,8 @@ extern "C" int hps_set_linearization(hps_operator op, const double *u)
     op->ops->fill(op->kp, op->n_tiles, u, op->cache64, op->cache32, op->stream);
   HPS_LAUNCHED();
   op->lin_set = true;
+  if (op->cut_tol >= 0.0)
+    return compute_cut(op, u);
   return HPS_OK;
 }
 
@@ -453,6 +520,8 @@ extern "C" int hps_set_constrained_nodes(hps_operator op, const int64_t *nodes, 
   HPS_HIP(hipStreamSynchronize(op->stream));
   (void)hipFree(op->constrained);
   (void)hipFree(op->constrained_save);
+  (void)hipFree(op->cut_masks);
+  (void)hipFree(op->cut_tiles);
   op->constrained      = nullptr;
   op->constrained_save = nullptr;
   op->n_constrained    = 0;
@@ -485,6 +554,14 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
   if (op->q2)
     op->q2->vmult64(q2_params(op), src, dst, op->cache64, op->stream);
+  else if (op->cut_ready)
+    for (int g = 0; g <= hps::CUT_MAX_GA; ++g)
+      {
+        const int count = op->cut_off[g + 1] - op->cut_off[g];
+        if (count > 0)
+          hps::vmult_cut_d3(g, op->kp, op->n_grains, op->cut_tiles + op->cut_off[g], count,
+                            op->cut_masks, src, op->cache64, dst, op->part, op->stream);
+      }
   else
     op->ops->vmult64(op->kp, op->n_tiles, src, dst, op->part, op->cache64, op->stream);
   HPS_LAUNCHED();
@@ -522,6 +599,15 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
     HPS_HIP(hipEventRecord(op->ev[0], op->stream));
   if (op->q2)
     op->q2->vmult32(q2_params(op), src, dst, op->cache32, op->stream);
+  else if (op->cut_ready)
+    for (int g = 0; g <= hps::CUT_MAX_GA; ++g)
+      {
+        const int count = op->cut_off[g + 1] - op->cut_off[g];
+        if (count > 0)
+          hps::vmult_cut_d3(g, op->kp, op->n_grains, op->cut_tiles + op->cut_off[g], count,
+                            op->cut_masks, src, op->cache32, dst, (float *)op->part,
+                            op->stream);
+      }
   else
     op->ops->vmult32(op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32,
                      op->stream);

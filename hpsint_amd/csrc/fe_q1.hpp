@@ -235,8 +235,13 @@ namespace hps
     int  pn_lane; // plane-node index of the lane's node
     bool cell_valid;
 
-    __device__ __forceinline__ Tile(const KParams &kp)
+    int tile; // index of the tile (workgroup of the full launch)
+
+    __device__ __forceinline__ Tile(const KParams &kp) : Tile(kp, (int)blockIdx.x) {}
+
+    __device__ __forceinline__ Tile(const KParams &kp, int tile_index)
     {
+      tile       = tile_index;
       tid        = threadIdx.x;
       corner     = tid & (NV - 1);
       const int cell = tid >> DIM;
@@ -245,7 +250,7 @@ namespace hps
       bx         = corner & 1;
       by         = DIM == 3 ? (corner >> 1) & 1 : 0;
       bm         = corner >> (DIM - 1);
-      int b      = blockIdx.x;
+      int b      = tile_index;
       const int tix = b % kp.nt[0];
       b /= kp.nt[0];
       const int tiy = b % kp.nt[1];
@@ -293,7 +298,7 @@ namespace hps
     if (!t.node_in_tile(nx, ny))
       return;
     if (t.shared(kp, nx, ny, k))
-      part[(((int64_t)blockIdx.x * (kp.tz + 1) + k) * TL::PN + pn) * nc + c] = v;
+      part[(((int64_t)t.tile * (kp.tz + 1) + k) * TL::PN + pn) * nc + c] = v;
     else
       dst[t.gnode(kp, nx, ny, k) * ostride + (int64_t)c * ocstride + ooffset] = v;
   }

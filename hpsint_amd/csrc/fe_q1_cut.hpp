@@ -1,0 +1,255 @@
+// fe_q1_cut.hpp -- grain cut-off for the Q1 tangent vmult (scalar mobility).
+//
+// The reference's cut-off (SinteringOperatorData::set_component_mask,
+// S/operator_sintering_data.h:150-298, and do_vmult_range_no_template,
+// S/operator_sintering_generic.h:456-537) evaluates each cell batch with
+// only the order parameters whose nodal values exceed the tolerance
+// somewhere in the batch: the others are dropped from the batch's physics
+// and receive no contribution from it.  Here the batch is a tile (the
+// TX x TY x tz cells one workgroup marches through): k_tile_masks sets bit
+// g of the tile's mask when |eta_g| > tol at some node of the tile, the
+// host groups the tiles by their number of active order parameters GA, and
+// q1_vmult_cut<GA> runs the tangent of a tile with GA + 2 components (c,
+// mu and the active eta), writing zeros to the inactive components of the
+// tile's nodes, so dst and the shared-node partials are complete for the
+// usual fix-up.
+#pragma once
+
+#include "fe_q1.hpp"
+
+namespace hps
+{
+  // bit g: |eta_g| > tol at some node of the tile (nodes of its cells)
+  template <int DIM, int TX, int TY>
+  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    k_tile_masks(const KParams kp, int G, double tol, const double *__restrict__ u,
+                 unsigned *__restrict__ masks)
+  {
+    using TL = Tile<DIM, TX, TY>;
+    const TL        t(kp);
+    const int       NC = G + 2;
+    __shared__ unsigned m_sh;
+    if (t.tid == 0)
+      m_sh = 0u;
+    __syncthreads();
+    unsigned  m     = 0u;
+    const int total = TL::PN * (t.tm_eff + 1);
+    for (int i = t.tid; i < total; i += TL::NT)
+      {
+        const int k = i / TL::PN, pn = i - k * TL::PN;
+        const int nx = pn % TL::PX, ny = pn / TL::PX;
+        if (!t.node_in_tile(nx, ny))
+          continue;
+        const double *x = u + t.gnode(kp, nx, ny, k) * NC + 2;
+        for (int g = 0; g < G; ++g)
+          if (fabs(x[g]) > tol)
+            m |= 1u << g;
+      }
+    atomicOr(&m_sh, m);
+    __syncthreads();
+    if (t.tid == 0)
+      masks[t.tile] = m_sh;
+  }
+
+  // global component of local component lc (0: c, 1: mu, 2 + s: the s-th
+  // active order parameter)
+  template <int NCA>
+  __device__ __forceinline__ int cut_component(const int (&gc)[NCA], int lc)
+  {
+    int r = 0;
+#pragma unroll
+    for (int s = 0; s < NCA; ++s)
+      r = lc == s ? gc[s] : r;
+    return r;
+  }
+
+  // tangent vmult of the tiles tiles[blockIdx.x], each with exactly GA
+  // active order parameters (scalar mobility, the folded form of q1_apply)
+  template <int DIM, int TX, int TY, int GA, typename T>
+  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    q1_vmult_cut(const KParams kp, int G, const int *__restrict__ tiles,
+                 const unsigned *__restrict__ masks, const T *__restrict__ src,
+                 const T *__restrict__ cache, T *__restrict__ dst, T *__restrict__ part)
+  {
+    using TL            = Tile<DIM, TX, TY>;
+    constexpr int NCA   = GA + 2;
+    constexpr int NT    = TL::NT;
+    constexpr int NV    = TL::NV;
+    constexpr int PN    = TL::PN;
+    constexpr int ITEMS = (PN * NCA + NT - 1) / NT;
+    constexpr int NFA   = 1 + GA + DIM; // local cached fields
+    const int     NC    = G + 2;
+    const int     NF    = 1 + G + DIM; // cached fields of the full state
+
+    __shared__ T ring[2][PN * NCA];
+    __shared__ T slots[NV][PN * NCA];
+
+    const TL       t(kp, tiles[blockIdx.x]);
+    const int      tid  = t.tid;
+    const unsigned mask = masks[t.tile];
+    int            gc[NCA];
+    gc[0] = 0;
+    gc[1] = 1;
+    {
+      unsigned m = mask;
+#pragma unroll
+      for (int s = 0; s < GA; ++s)
+        {
+          gc[2 + s] = 2 + (m ? __builtin_ctz(m) : 0);
+          m &= m - 1u;
+        }
+    }
+
+    const T jxw = T(kp.jxw);
+    T       gsj[DIM];
+    {
+      const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+      signed_scales<DIM>(t.corner, inv_h, T(kp.jxw), gsj);
+    }
+    FoldScales<T> fs;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        fs.kj[d]   = T(kp.fold_kj[d]);
+        fs.kmu[d]  = T(kp.fold_kmu[d]);
+        fs.keta[d] = T(kp.fold_keta[d]);
+        fs.gsj[d]  = gsj[d];
+      }
+
+    T    regs[ITEMS];
+    auto load_plane = [&](int k) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        {
+          const int i = tid + r * NT;
+          if (i < PN * NCA)
+            {
+              const int pn = i / NCA, lc = i - pn * NCA;
+              const int nx = pn % TL::PX, ny = pn / TL::PX;
+              T         v  = T(0);
+              if (t.node_in_tile(nx, ny))
+                v = src[t.gnode(kp, nx, ny, k) * NC + cut_component<NCA>(gc, lc)];
+              regs[r] = v;
+            }
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        {
+          const int i = tid + r * NT;
+          if (i < PN * NCA)
+            ring[slot][i] = regs[r];
+        }
+    };
+    // zeros for the inactive components of plane k's nodes
+    auto put_zeros = [&](int k) {
+      for (int i = tid; i < PN * NC; i += NT)
+        {
+          const int pn = i / NC, c = i - pn * NC;
+          if (c >= 2 && !((mask >> (c - 2)) & 1u))
+            put(kp, t, NC, k, pn, c, T(0), dst, part, NC, 1, 0);
+        }
+    };
+
+    T carry[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      carry[r] = T(0);
+    load_plane(0);
+    store_plane(0);
+    load_plane(1);
+    store_plane(1);
+    __syncthreads();
+
+    const T *cache_tile = cache + (int64_t)t.tile * kp.tz * NF * NT;
+    for (int k = 0; k < t.tm_eff; ++k)
+      {
+        const bool pre = k + 2 <= t.tm_eff;
+        if (pre)
+          load_plane(k + 2);
+
+        const T *pl = &ring[(k + t.bm) & 1][t.pn_lane * NCA];
+        T        val[NCA], grad[NCA][DIM];
+#pragma unroll
+        for (int c = 0; c < NCA; ++c)
+          interp_vg_pre<DIM>(pl[c], val[c], grad[c]);
+
+        T *sl = &slots[t.corner][t.pn_lane * NCA];
+        {
+          // the cache fields of the active components: c, eta_g, grad mu
+          T        lin[NFA];
+          const T *cl = cache_tile + k * (NF * NT) + tid;
+          lin[0]      = cl[0];
+#pragma unroll
+          for (int s = 0; s < GA; ++s)
+            lin[1 + s] = cl[(gc[2 + s] - 1) * NT];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            lin[1 + GA + d] = cl[(1 + G + d) * NT];
+          using CL = CacheLayout<DIM, GA, false>;
+          T vr[NCA], gr[NCA][DIM];
+          tangent_qp_rows<DIM, GA, false, true, false>(kp.pp, lin, val, grad, vr, gr);
+          T M, sm;
+          scalar_mobility_terms<GA>(kp.pp, lin[CL::c], lin + CL::eta, val, M, sm);
+          const T *lmu = lin + CL::grad_mu;
+#pragma unroll
+          for (int c = 0; c < NCA; ++c)
+            {
+              T gs[DIM];
+#pragma unroll
+              for (int d = 0; d < DIM; ++d)
+                gs[d] = c == 0 ? (M * fs.kj[d]) * grad[1][d] + sm * (lmu[d] * fs.gsj[d])
+                               : (c == 1 ? fs.kmu[d] * grad[0][d] : fs.keta[d] * grad[c][d]);
+              sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
+            }
+        }
+        __syncthreads();
+        if (pre)
+          store_plane(k & 1);
+
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+          {
+            const int i = tid + r * NT;
+            if (i < PN * NCA)
+              {
+                const int pn = i / NCA, lc = i - pn * NCA;
+                const int nx = pn % TL::PX, ny = pn / TL::PX;
+                T         fin = carry[r], nxt = T(0);
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                  {
+                    const int qx = q & 1, qy = DIM == 3 ? (q >> 1) & 1 : 0;
+                    const int qm = q >> (DIM - 1);
+                    const int ccx = nx - qx, ccy = ny - qy;
+                    if (ccx >= 0 && ccx < t.tx_eff && ccy >= 0 && ccy < t.ty_eff)
+                      {
+                        const T v = slots[q][i];
+                        if (qm == 0)
+                          fin += v;
+                        else
+                          nxt += v;
+                      }
+                  }
+                carry[r] = nxt;
+                put(kp, t, NC, k, pn, cut_component<NCA>(gc, lc), fin, dst, part, NC, 1, 0);
+              }
+          }
+        put_zeros(k);
+        __syncthreads();
+      }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      {
+        const int i = tid + r * NT;
+        if (i < PN * NCA)
+          {
+            const int pn = i / NCA, lc = i - pn * NCA;
+            put(kp, t, NC, t.tm_eff, pn, cut_component<NCA>(gc, lc), carry[r], dst, part, NC,
+                1, 0);
+          }
+      }
+    put_zeros(t.tm_eff);
+  }
+} // namespace hps

@@ -216,4 +216,17 @@ namespace hps
 
   template <typename T>
   void launch_fixup(const FixParams &fp, const T *part, T *dst, hipStream_t s);
+
+  // grain cut-off (3D, scalar mobility, TX3 x TY3 tiles): per-tile masks of
+  // the active order parameters, and the tangent vmult of `count` tiles
+  // (tiles[]) that all have ga active order parameters
+  constexpr int CUT_MAX_GA = 8;
+  void tile_masks_d3(const KParams &kp, int n_tiles, int n_grains, double tol, const double *u,
+                     unsigned *masks, hipStream_t s);
+  bool vmult_cut_d3(int ga, const KParams &kp, int n_grains, const int *tiles, int count,
+                    const unsigned *masks, const double *src, const double *cache, double *dst,
+                    double *part, hipStream_t s);
+  bool vmult_cut_d3(int ga, const KParams &kp, int n_grains, const int *tiles, int count,
+                    const unsigned *masks, const float *src, const float *cache, float *dst,
+                    float *part, hipStream_t s);
 } // namespace hps

@@ -245,6 +245,13 @@ class SinteringOperatorGeneric:
         check(lib().hps_set_constrained_nodes(
             self._h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(arr.size)))
 
+    def set_grain_cut_off(self, tol):
+        """grain cut-off of the tangent (set_component_mask,
+        S/operator_sintering_data.h:150-298): per tile, the order parameters
+        with |eta| <= tol at all its nodes are dropped from the tangent;
+        takes effect at the next fill_quadrature_point_values; tol < 0: off"""
+        check(lib().hps_set_grain_cut_off(self._h, float(tol)))
+
     def vmult(self, dst, src):
         if src.dtype == torch.float32:
             check(lib().hps_vmult_f32(self._h, _ptr(dst), _ptr(src)))

@@ -120,6 +120,17 @@ int hps_residual(hps_operator op, int wtd, double *dst, const double *src,
  * preconditioners use diagonal 1 and identity node blocks for them. */
 int hps_set_constrained_nodes(hps_operator op, const int64_t *nodes, int64_t n);
 
+/* Grain cut-off of the tangent vmult: SinteringOperatorData::
+ * set_component_mask (S/operator_sintering_data.h:150-298) with
+ * do_vmult_range_no_template (S/operator_sintering_generic.h:456-537).
+ * tol >= 0: at each hps_set_linearization the order parameters with
+ * |eta| <= tol at every node of a tile (the cells one workgroup processes:
+ * 8 x 4 cells x the march length) are dropped from that tile's part of the
+ * tangent; tol < 0: off (the default).  Q1, 3D, scalar mobility; a tile with
+ * more than 8 active order parameters makes that linearisation use the full
+ * vmult.  The residual is never cut. */
+int hps_set_grain_cut_off(hps_operator op, double tol);
+
 /* OperatorBase::vmult -> SinteringOperatorGeneric::vmult_internal
  * (S/operator_base.h:294-353, S/operator_sintering_generic.h:579-606):
  * tangent at the linearisation point; dst is overwritten.  The f32 variant

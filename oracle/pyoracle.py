@@ -220,6 +220,57 @@ def residual_constrained(p, wtd, src, old, nodes):
     return dst.ravel()
 
 
+def vmult_cut(p, lin, src, cell_masks):
+    """tangent vmult with a grain cut-off, do_vmult_range_no_template
+    (S/operator_sintering_generic.h:456-537): each cell is evaluated with c,
+    mu and only the order parameters whose bit is set in cell_masks[cell]
+    (cells x fastest); the others get no contribution from the cell and do
+    not enter its physics.  Q1, 3D, 2-point Gauss; the quadrature-point
+    tangent is orc_tangent_qp on the reduced component set."""
+    assert p.dim == 3 and p.degree == 1
+    nc = p.n_comp
+    ncx, ncy, ncz = p.n_cells[0], p.n_cells[1], p.n_cells[2]
+    h = [p.h[0], p.h[1], p.h[2]]
+    nnx, nny = ncx + 1, ncy + 1
+    lin = np.asarray(lin, dtype=np.float64).reshape(-1, nc)
+    src = np.asarray(src, dtype=np.float64).reshape(-1, nc)
+    dst = np.zeros_like(src)
+    xg = [(1.0 - 1.0 / np.sqrt(3.0)) / 2.0, (1.0 + 1.0 / np.sqrt(3.0)) / 2.0]
+    corners = [(a, b, c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+    # shape values and gradients of the 8 corners at the 8 points
+    S = np.zeros((8, 8))
+    Gr = np.zeros((8, 8, 3))
+    for q, (qa, qb, qc) in enumerate(corners):
+        x = (xg[qa], xg[qb], xg[qc])
+        for k, (a, b, c) in enumerate(corners):
+            f = [x[d] if (a, b, c)[d] else 1.0 - x[d] for d in range(3)]
+            df = [(1.0 if (a, b, c)[d] else -1.0) / h[d] for d in range(3)]
+            S[q, k] = f[0] * f[1] * f[2]
+            Gr[q, k] = [df[0] * f[1] * f[2], f[0] * df[1] * f[2], f[0] * f[1] * df[2]]
+    jxw = h[0] * h[1] * h[2] / 8.0
+    for cz in range(ncz):
+        for cy in range(ncy):
+            for cx in range(ncx):
+                cell = cx + ncx * (cy + ncy * cz)
+                m = int(cell_masks[cell])
+                L = [0, 1] + [2 + g for g in range(nc - 2) if (m >> g) & 1]
+                p2 = type(p).from_buffer_copy(p)
+                p2.n_comp = len(L)
+                nodes = [(cx + a) + nnx * ((cy + b) + nny * (cz + c)) for (a, b, c) in corners]
+                ul = lin[nodes][:, L]
+                us = src[nodes][:, L]
+                for q in range(8):
+                    lv = S[q] @ ul
+                    lg = (Gr[q].T @ ul).T
+                    v = S[q] @ us
+                    g = (Gr[q].T @ us).T
+                    vr, gr = tangent_qp(p2, lv, lg, v, g)
+                    contrib = jxw * (np.outer(S[q], vr) + Gr[q] @ gr.T)
+                    for k, nd in enumerate(nodes):
+                        dst[nd, L] += contrib[k]
+    return dst.ravel()
+
+
 def node_blocks(p, cache):
     nn = n_nodes(p)
     out = np.empty(nn * p.n_comp * p.n_comp)
