@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--train-iterations", type=int, default=6000)
     ap.add_argument("--q2-cells", type=int, default=96,
                     help="cells per direction of the Q2 tensorial workload (0 = skip)")
+    ap.add_argument("--cut-tol", type=float, default=1e-5,
+                    help="grain cut-off tolerance timed on the 27-sphere workload (<0: not)")
     ap.add_argument("--q2-timesteps", type=int, default=2,
                     help="timed Newton steps per method on the Q2 workload (0: none)")
     ap.add_argument("--mp-cells", type=int, default=192,
@@ -196,6 +198,24 @@ def mp_vmult(args, timed):
            "vmult_f32_gdofs": op.n_dofs / t32 / 1e9,
            "vmult_f32_rel_l2_error_per_field": err,
            "residual_f64_gdofs": op.n_dofs / t_res / 1e9}
+    if args.cut_tol >= 0.0:
+        # the grain cut-off of the tangent: order parameters below the
+        # tolerance on a whole tile are dropped from that tile's tangent
+        full = dst.clone()
+        op.set_grain_cut_off(args.cut_tol)
+        op.fill_quadrature_point_values(u)
+        for _ in range(3):
+            op.vmult(dst, src)
+        tc = timed(lambda: op.vmult(dst, src), k)
+        tc32 = timed(lambda: op.vmult(dst32, src32), k)
+        op.vmult(dst, src)
+        out["cut_off"] = {
+            "tol": args.cut_tol, "vmult_f64_gdofs": op.n_dofs / tc / 1e9,
+            "vmult_f64_ms": tc * 1e3, "vmult_f32_gdofs": op.n_dofs / tc32 / 1e9,
+            "speedup_vs_full_f64": t64 / tc,
+            "rel_l2_change_vs_full_tangent": float((dst - full).norm() / full.norm())}
+        op.set_grain_cut_off(-1.0)
+        del full
     del op, u, src, dst, src32, dst32, r, old
     torch.cuda.empty_cache()
     return out
