@@ -115,9 +115,15 @@ def predictor_study(args):
         assert pk.n_order_params <= G
         return torch.from_numpy(packing.initial_fields(pk, (n, n, n), n_order_params=G)).cuda()
 
+    def progress(msg):
+        print("predictor study: " + msg, file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
-    trajs = [collect_trajectory(op, fields(s), n_steps=args.study_train_steps, dt=dt,
-                                nl_rel_tol=1e-8) for s in range(args.study_seeds)]
+    trajs = []
+    for s in range(args.study_seeds):
+        trajs.append(collect_trajectory(op, fields(s), n_steps=args.study_train_steps, dt=dt,
+                                        nl_rel_tol=1e-8))
+        progress("training packing %d: %d plain steps done" % (s, args.study_train_steps))
     t_traj = time.perf_counter() - t0
     t0 = time.perf_counter()
     pred = fit_predictor(op, trajs, dt, iterations=args.train_iterations,
@@ -125,6 +131,7 @@ def predictor_study(args):
                          column_weights=(3.0, 1.0, 1.0, 1.0))
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
+    progress("network trained in %.1f s" % t_fit)
     del trajs
     torch.cuda.empty_cache()
     held = collect_trajectory(op, fields(args.study_seeds), n_steps=3, dt=dt, nl_rel_tol=1e-8)
@@ -138,6 +145,8 @@ def predictor_study(args):
         s.history = [held[3].clone(), held[2].clone()]
         s.t = 3 * dt
         rs = [s.make_step() for _ in range(args.study_test_steps)]
+        progress("held-out steps done (fp32 inner %s, predictor %s)"
+                 % (fp32, predictor is not None))
         return {"s_per_step": sum(r.seconds for r in rs) / len(rs),
                 "newton": [r.newton_iterations for r in rs],
                 "linear": [r.linear_iterations for r in rs]}
