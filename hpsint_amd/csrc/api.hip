@@ -207,7 +207,7 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   kp.pp.w0      = 0.0;
   kp.pp.w_alg   = 1.0;
   kp.order      = 0;
-  hps::set_fold_scales(kp);
+  hps::set_fold_scales(kp, prm->dim);
 
   op->n_nodes = (int64_t)kp.nn[0] * kp.nn[1] * kp.nn[2];
   op->n_dofs  = op->n_nodes * op->nc;

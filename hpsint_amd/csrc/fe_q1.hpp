@@ -51,20 +51,28 @@ namespace hps
     double jxw;       // prod(h) * (1/2)^DIM
     double w_hist[3]; // BDF weights[1..order]
     PhysParams pp;
-    // uniform scales of the folded scalar tangent (FoldScales, set by
-    // set_fold_scales): JxW/h^2, times -kappa_c, times L kappa_p
-    double fold_kj[3], fold_kmu[3], fold_keta[3];
+    // uniform scales of the folded scalar tangent for the rescaled
+    // butterflies (FoldScales, set by set_fold_scales; sa = (1+1/sqrt3)/2):
+    // JxW/h^2 sa^(2 DIM - 2), times -kappa_c, times L kappa_p;
+    // JxW/h sa^(2 DIM - 1) (signed per lane by the kernels); JxW sa^(2 DIM)
+    double fold_kj[3], fold_kmu[3], fold_keta[3], fold_gs[3], fold_jxw;
   };
 
   // fills the uniform folded-tangent scales of kp from jxw, inv_h and pp
-  inline void set_fold_scales(KParams &kp)
+  inline void set_fold_scales(KParams &kp, int dim)
   {
+    const double sa = 0.78867513459481288225457439025098;
+    double       eg = 1.0;
+    for (int i = 0; i < 2 * dim - 2; ++i)
+      eg *= sa;
     for (int d = 0; d < 3; ++d)
       {
-        kp.fold_kj[d]   = kp.jxw * kp.inv_h[d] * kp.inv_h[d];
+        kp.fold_kj[d]   = kp.jxw * kp.inv_h[d] * kp.inv_h[d] * eg;
         kp.fold_kmu[d]  = -kp.pp.kappa_c * kp.fold_kj[d];
         kp.fold_keta[d] = kp.pp.L * kp.pp.kappa_p * kp.fold_kj[d];
+        kp.fold_gs[d]   = kp.jxw * kp.inv_h[d] * (eg * sa);
       }
+    kp.fold_jxw = kp.jxw * (eg * sa * sa);
   }
 
   enum Op
@@ -209,6 +217,93 @@ namespace hps
       }
   }
 
+  // ----- rescaled butterflies (one FMA per value level)
+  // With r = sb / sa = 2 - sqrt(3), a value level sa x + sb x_partner is
+  // sa (x + r x_partner); the factor sa of each level is left to constants
+  // the caller folds.
+  //   interp_vg_hat: val = sa^DIM val_hat, grad_d = sa^(DIM-1) grad_hat_d
+  //     (unsigned, unscaled derivatives as interp_vg_pre);
+  //   integrate_vg_hat(sa^DIM v, sa^(DIM-1) g) = integrate_vg(v, g).
+  template <typename T>
+  struct Q1Hat
+  {
+    static constexpr T r = T(0.26794919243112270647255365849413);
+  };
+
+  template <int DIM, typename T>
+  __device__ __forceinline__ void interp_vg_hat(T u, T &val, T *grad)
+  {
+    const T r = Q1Hat<T>::r;
+    if constexpr (DIM == 2)
+      {
+        const T p  = xshfl<1>(u);
+        const T A  = u + r * p;
+        const T Dx = u - p;
+        const T pA = xshfl<2>(A);
+        const T pD = xshfl<2>(Dx);
+        val        = A + r * pA;
+        grad[1]    = A - pA;
+        grad[0]    = Dx + r * pD;
+      }
+    else
+      {
+        const T p    = xshfl<4>(u);
+        const T A    = u + r * p;
+        const T Dz   = u - p;
+        const T pA   = xshfl<2>(A);
+        const T pDz  = xshfl<2>(Dz);
+        const T AA   = A + r * pA;
+        const T Ay   = A - pA;
+        const T DzA  = Dz + r * pDz;
+        const T pAA  = xshfl<1>(AA);
+        const T pAy  = xshfl<1>(Ay);
+        const T pDzA = xshfl<1>(DzA);
+        val          = AA + r * pAA;
+        grad[0]      = AA - pAA;
+        grad[1]      = Ay + r * pAy;
+        grad[2]      = DzA + r * pDzA;
+      }
+  }
+
+  template <int DIM, typename T>
+  __device__ __forceinline__ T integrate_vg_hat(T v, const T *g)
+  {
+    const T r = Q1Hat<T>::r;
+    if constexpr (DIM == 2)
+      {
+        const T U1 = (v + g[1]) + xshfl<2>(r * v - g[1]);
+        const T Ux = g[0] + r * xshfl<2>(g[0]);
+        return (U1 + Ux) + xshfl<1>(r * U1 - Ux);
+      }
+    else
+      {
+        const T T1 = (v + g[0]) + xshfl<1>(r * v - g[0]);
+        const T Ty = g[1] + r * xshfl<1>(g[1]);
+        const T Tz = g[2] + r * xshfl<1>(g[2]);
+        const T U1 = (T1 + Ty) + xshfl<2>(r * T1 - Ty);
+        const T Uz = Tz + r * xshfl<2>(Tz);
+        return (U1 + Uz) + xshfl<4>(r * U1 - Uz);
+      }
+  }
+
+  // FoldScales of the scalar-mobility tangent for the rescaled butterflies
+  // (the scales of KParams, the mobility-derivative one signed by the lane's
+  // corner) and the value-row factor jxw_hat
+  template <int DIM, typename T>
+  __device__ __forceinline__ void fold_scales_hat(const KParams &kp, int corner,
+                                                  FoldScales<T> &fs, T &jxw_hat)
+  {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      {
+        fs.kj[d]   = T(kp.fold_kj[d]);
+        fs.kmu[d]  = T(kp.fold_kmu[d]);
+        fs.keta[d] = T(kp.fold_keta[d]);
+        fs.gsj[d]  = ((corner >> d) & 1) ? T(kp.fold_gs[d]) : -T(kp.fold_gs[d]);
+      }
+    jxw_hat = T(kp.fold_jxw);
+  }
+
   // sum over the 2^DIM lanes of a cell
   template <int DIM, typename T>
   __device__ __forceinline__ T cell_sum(T v)
@@ -303,7 +398,6 @@ namespace hps
       dst[t.gnode(kp, nx, ny, k) * ostride + (int64_t)c * ocstride + ooffset] = v;
   }
 
-  // waves per SIMD the register allocation of q1_apply aims at
   // ------------------------------------------- vmult / residual kernel
   template <int DIM, int TX, int TY, int G, bool TENS, int OP, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
@@ -335,16 +429,12 @@ namespace hps
     const T jxw = T(kp.jxw);
     // scalar-mobility tangent: constant gradient rows folded into uniform
     // scales (FoldScales)
+    // (rescaled butterflies: interp_vg_hat / integrate_vg_hat)
     constexpr bool FOLD = !TENS && OP == OP_VMULT;
     FoldScales<T>  fs;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      {
-        fs.kj[d]   = T(kp.fold_kj[d]);
-        fs.kmu[d]  = T(kp.fold_kmu[d]);
-        fs.keta[d] = T(kp.fold_keta[d]);
-        fs.gsj[d]  = gsj[d];
-      }
+    T              jxw_hat = T(0);
+    if constexpr (FOLD)
+      fold_scales_hat<DIM>(kp, t.corner, fs, jxw_hat);
 
     T regs[NPL][ITEMS];
     auto load_plane = [&](int k) {
@@ -417,7 +507,7 @@ namespace hps
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           if constexpr (FOLD)
-            interp_vg_pre<DIM>(pl[c], val[c], grad[c]);
+            interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
           else
             interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
 
@@ -444,7 +534,7 @@ namespace hps
                 for (int d = 0; d < DIM; ++d)
                   gs[d] = c == 0 ? (M * fs.kj[d]) * grad[1][d] + s * (lmu[d] * fs.gsj[d])
                                  : (c == 1 ? fs.kmu[d] * grad[0][d] : fs.keta[d] * grad[c][d]);
-                sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
+                sl[c] = integrate_vg_hat<DIM>(vr[c] * jxw_hat, gs);
               }
           }
         else if constexpr (OP == OP_VMULT)

@@ -64,7 +64,8 @@ namespace hps
   }
 
   // tangent vmult of the tiles tiles[blockIdx.x], each with exactly GA
-  // active order parameters (scalar mobility, the folded form of q1_apply)
+  // active order parameters (scalar mobility, the folded form of q1_apply
+  // with the rescaled butterflies)
   template <int DIM, int TX, int TY, int GA, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
     q1_vmult_cut(const KParams kp, int G, const int *__restrict__ tiles,
@@ -100,21 +101,9 @@ namespace hps
         }
     }
 
-    const T jxw = T(kp.jxw);
-    T       gsj[DIM];
-    {
-      const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
-      signed_scales<DIM>(t.corner, inv_h, T(kp.jxw), gsj);
-    }
     FoldScales<T> fs;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-      {
-        fs.kj[d]   = T(kp.fold_kj[d]);
-        fs.kmu[d]  = T(kp.fold_kmu[d]);
-        fs.keta[d] = T(kp.fold_keta[d]);
-        fs.gsj[d]  = gsj[d];
-      }
+    T             jxw_hat;
+    fold_scales_hat<DIM>(kp, t.corner, fs, jxw_hat);
 
     T    regs[ITEMS];
     auto load_plane = [&](int k) {
@@ -173,7 +162,7 @@ namespace hps
         T        val[NCA], grad[NCA][DIM];
 #pragma unroll
         for (int c = 0; c < NCA; ++c)
-          interp_vg_pre<DIM>(pl[c], val[c], grad[c]);
+          interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
 
         T *sl = &slots[t.corner][t.pn_lane * NCA];
         {
@@ -201,7 +190,7 @@ namespace hps
               for (int d = 0; d < DIM; ++d)
                 gs[d] = c == 0 ? (M * fs.kj[d]) * grad[1][d] + sm * (lmu[d] * fs.gsj[d])
                                : (c == 1 ? fs.kmu[d] * grad[0][d] : fs.keta[d] * grad[c][d]);
-              sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
+              sl[c] = integrate_vg_hat<DIM>(vr[c] * jxw_hat, gs);
             }
         }
         __syncthreads();
