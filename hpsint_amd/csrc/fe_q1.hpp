@@ -399,8 +399,18 @@ namespace hps
   }
 
   // ------------------------------------------- vmult / residual kernel
+  // waves per SIMD the register allocation of q1_apply is asked to allow:
+  // 3 (at most 168 registers) for the FP64 scalar tangent with up to 4
+  // order parameters, whose allocation is close to that limit
+  template <int G, bool TENS, int OP, typename T>
+  constexpr int q1_apply_waves()
+  {
+    return (sizeof(T) == 8 && !TENS && OP == OP_VMULT && G <= 4) ? 3 : 1;
+  }
+
   template <int DIM, int TX, int TY, int G, bool TENS, int OP, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    __attribute__((amdgpu_waves_per_eu(q1_apply_waves<G, TENS, OP, T>())))
     q1_apply(const KParams kp, const T *__restrict__ src, const T *__restrict__ old0,
              const T *__restrict__ old1, const T *__restrict__ old2,
              const T *__restrict__ cache, T *__restrict__ dst, T *__restrict__ part)
@@ -494,11 +504,26 @@ namespace hps
 
     // uniform base of this tile's cache + a 32-bit per-lane offset
     const T *cache_tile = cache + (int64_t)blockIdx.x * kp.tz * NF * NT;
+    // PF (3D, up to 4 order parameters: where the registers it holds do
+    // not lower the occupancy): the next layer's cache fields and the plane
+    // after next are requested together at the end of a layer, so that the
+    // layer's waits do not cover them and both arrive during the node sums
+    // and the next interpolation
+    constexpr bool PF = FOLD && DIM == 3 && G <= 4;
+    T              lin_next[PF ? NF : 1];
+    if constexpr (PF)
+      {
+        if (2 <= t.tm_eff)
+          load_plane(2);
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          lin_next[f] = cache_tile[f * NT + tid];
+      }
 
     for (int k = 0; k < t.tm_eff; ++k)
       {
         const bool pre = k + 2 <= t.tm_eff;
-        if (pre)
+        if (!PF && pre)
           load_plane(k + 2);
 
         // ---- cell layer k: interpolate, physics, integrate
@@ -510,6 +535,8 @@ namespace hps
             interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
           else
             interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
+        if constexpr (PF)
+          __builtin_amdgcn_sched_barrier(0); // the physics waits for lin_next
 
         T vr[NC], gr[NC][DIM];
         T *sl = &slots[t.corner][t.pn_lane * NC];
@@ -518,14 +545,21 @@ namespace hps
             // value rows first, then each component's gradient row is
             // formed and integrated at once (shorter live ranges)
             T lin[NF];
-            const T *cl = cache_tile + k * (NF * NT);
+            if constexpr (PF)
 #pragma unroll
-            for (int f = 0; f < NF; ++f)
-              lin[f] = cl[f * NT + tid];
+              for (int f = 0; f < NF; ++f)
+                lin[f] = lin_next[f];
+            else
+#pragma unroll
+              for (int f = 0; f < NF; ++f)
+                lin[f] = cache_tile[k * (NF * NT) + f * NT + tid];
             tangent_qp_rows<DIM, G, false, true, false>(kp.pp, lin, val, grad, vr, gr);
             T M, s;
             scalar_mobility_terms<G>(kp.pp, lin[CL::c], lin + CL::eta, val, M, s);
-            const T *lmu = lin + CL::grad_mu;
+            T lmu[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              lmu[d] = lin[CL::grad_mu + d];
 #pragma unroll
             for (int c = 0; c < NC; ++c)
               {
@@ -573,6 +607,19 @@ namespace hps
 
         if (pre)
           store_plane(k & 1);
+
+        if constexpr (PF)
+          {
+            if (k + 3 <= t.tm_eff)
+              load_plane(k + 3);
+            if (k + 1 < t.tm_eff)
+              {
+                const T *cn = cache_tile + (k + 1) * (NF * NT);
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                  lin_next[f] = cn[f * NT + tid];
+              }
+          }
 
         // ---- node sums: plane k is complete, plane k+1 begins
 #pragma unroll
