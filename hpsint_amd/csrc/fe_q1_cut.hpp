@@ -68,6 +68,7 @@ namespace hps
   // with the rescaled butterflies)
   template <int DIM, int TX, int TY, int GA, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && GA == 1 ? 5 : 1)))
     q1_vmult_cut(const KParams kp, int G, const int *__restrict__ tiles,
                  const unsigned *__restrict__ masks, const T *__restrict__ src,
                  const T *__restrict__ cache, T *__restrict__ dst, T *__restrict__ part)
@@ -151,37 +152,51 @@ namespace hps
     store_plane(1);
     __syncthreads();
 
+    // the cache fields of the active components of layer k: c, eta_g,
+    // grad mu.  As in q1_apply, the next layer's fields and the plane after
+    // next are requested together at the end of a layer.
     const T *cache_tile = cache + (int64_t)t.tile * kp.tz * NF * NT;
+    T        lin_next[NFA];
+    auto     load_lin = [&](int k) {
+      const T *cl = cache_tile + k * (NF * NT) + tid;
+      lin_next[0] = cl[0];
+#pragma unroll
+      for (int s = 0; s < GA; ++s)
+        lin_next[1 + s] = cl[(gc[2 + s] - 1) * NT];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        lin_next[1 + GA + d] = cl[(1 + G + d) * NT];
+    };
+    if (2 <= t.tm_eff)
+      load_plane(2);
+    load_lin(0);
+
     for (int k = 0; k < t.tm_eff; ++k)
       {
         const bool pre = k + 2 <= t.tm_eff;
-        if (pre)
-          load_plane(k + 2);
 
         const T *pl = &ring[(k + t.bm) & 1][t.pn_lane * NCA];
         T        val[NCA], grad[NCA][DIM];
 #pragma unroll
         for (int c = 0; c < NCA; ++c)
           interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
+        __builtin_amdgcn_sched_barrier(0); // the physics waits for lin_next
 
         T *sl = &slots[t.corner][t.pn_lane * NCA];
         {
-          // the cache fields of the active components: c, eta_g, grad mu
-          T        lin[NFA];
-          const T *cl = cache_tile + k * (NF * NT) + tid;
-          lin[0]      = cl[0];
+          T lin[NFA];
 #pragma unroll
-          for (int s = 0; s < GA; ++s)
-            lin[1 + s] = cl[(gc[2 + s] - 1) * NT];
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            lin[1 + GA + d] = cl[(1 + G + d) * NT];
+          for (int f = 0; f < NFA; ++f)
+            lin[f] = lin_next[f];
           using CL = CacheLayout<DIM, GA, false>;
           T vr[NCA], gr[NCA][DIM];
           tangent_qp_rows<DIM, GA, false, true, false>(kp.pp, lin, val, grad, vr, gr);
           T M, sm;
           scalar_mobility_terms<GA>(kp.pp, lin[CL::c], lin + CL::eta, val, M, sm);
-          const T *lmu = lin + CL::grad_mu;
+          T lmu[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            lmu[d] = lin[CL::grad_mu + d];
 #pragma unroll
           for (int c = 0; c < NCA; ++c)
             {
@@ -196,6 +211,10 @@ namespace hps
         __syncthreads();
         if (pre)
           store_plane(k & 1);
+        if (k + 3 <= t.tm_eff)
+          load_plane(k + 3);
+        if (k + 1 < t.tm_eff)
+          load_lin(k + 1);
 
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r)
