@@ -535,8 +535,10 @@ namespace hps
             interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
           else
             interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
-        if constexpr (PF)
-          __builtin_amdgcn_sched_barrier(0); // the physics waits for lin_next
+        // the physics stays below the interpolation (with PF it waits for
+        // lin_next; in 3D it also shortens the register live ranges)
+        if constexpr (FOLD && DIM == 3)
+          __builtin_amdgcn_sched_barrier(0);
 
         T vr[NC], gr[NC][DIM];
         T *sl = &slots[t.corner][t.pn_lane * NC];
