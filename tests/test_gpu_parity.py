@@ -67,6 +67,9 @@ VMULT_CASES = [
     (3, (19, 13, 37), 4, False),
     (3, (8, 4, 16), 1, False),
     (3, (3, 7, 2), 6, False),
+    (3, (10, 6, 19), 5, False),
+    (3, (9, 5, 17), 8, False),
+    (3, (7, 5, 18), 12, False),
     (3, (11, 6, 18), 3, True),
     (3, (5, 9, 20), 2, True),
     (2, (65, 17), 2, False),
@@ -148,7 +151,8 @@ def test_residual_golden_2cell_on_gpu(golden_dir):
 
 
 @pytest.mark.parametrize("dim,cells,G,tens", [
-    (3, (9, 5, 17), 2, False), (3, (5, 9, 20), 2, True), (2, (65, 17), 3, False)])
+    (3, (9, 5, 17), 2, False), (3, (5, 9, 20), 2, True), (2, (65, 17), 3, False),
+    (3, (10, 6, 19), 4, False), (3, (7, 5, 18), 12, False)])
 def test_vmult_f32_error_per_field(dim, cells, G, tens):
     """FP32 vmult (FP32 cache, FP32 arithmetic) against the FP64 oracle;
     bound: per-field relative l2 error <= 1e-5"""
