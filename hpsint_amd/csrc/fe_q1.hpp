@@ -1029,6 +1029,21 @@ namespace hps
     int nb[3];                       // interior tile boundaries per direction
     int64_t n_m, n_y, n_x;           // thread counts of the three classes
     int ostride, ocstride, ooffset;
+    int vdst;                        // dst node records written as vectors
+  };
+
+  // the widest load (16 or 8 bytes) that tiles a record of NC values of T
+  template <typename T, int NC>
+  struct FixVec
+  {
+    static constexpr int bytes = (NC * sizeof(T)) % 16 == 0 ? 16
+                                 : (NC * sizeof(T)) % 8 == 0 ? 8
+                                                              : (int)sizeof(T);
+    static constexpr int w     = bytes / (int)sizeof(T);
+    struct alignas(bytes) V
+    {
+      T x[w];
+    };
   };
 
   __device__ __forceinline__ bool is_bnd(int g, int tsz, int nc)
@@ -1097,6 +1112,7 @@ namespace hps
             lo[d][1] = lo[d][0];
           }
       }
+    using FV           = FixVec<T, NC>;
     const int64_t node = ((int64_t)g[2] * fp.nn[1] + g[1]) * fp.nn[0] + g[0];
     T             s[NC];
 #pragma unroll
@@ -1109,10 +1125,30 @@ namespace hps
             const int64_t tile = ((int64_t)tl[2][a] * fp.nt[1] + tl[1][b]) * fp.nt[0] + tl[0][e];
             const int     pn   = lo[0][e] + fp.px * lo[1][b];
             const T      *p    = part + ((tile * (fp.tsz[2] + 1) + lo[2][a]) * fp.pn + pn) * NC;
+            const auto   *pv   = reinterpret_cast<const typename FV::V *>(p);
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-              s[c] += p[c];
+            for (int v = 0; v < NC / FV::w; ++v)
+              {
+                const typename FV::V x = pv[v];
+#pragma unroll
+                for (int j = 0; j < FV::w; ++j)
+                  s[v * FV::w + j] += x.x[j];
+              }
           }
+    if (fp.vdst)
+      {
+        auto *dv = reinterpret_cast<typename FV::V *>(dst + node * NC);
+#pragma unroll
+        for (int v = 0; v < NC / FV::w; ++v)
+          {
+            typename FV::V x;
+#pragma unroll
+            for (int j = 0; j < FV::w; ++j)
+              x.x[j] = s[v * FV::w + j];
+            dv[v] = x;
+          }
+        return;
+      }
     T *d = dst + node * fp.ostride + fp.ooffset;
 #pragma unroll
     for (int c = 0; c < NC; ++c)

@@ -50,11 +50,19 @@ namespace hps
     if (n == 0)
       return;
     const int blocks = (int)((n + 255) / 256);
+    // node-interleaved dst (the vmult and residual layout) of suitable
+    // alignment: records stored as vectors
+    auto with_vdst = [&](int bytes) {
+      FixParams f = fp;
+      f.vdst      = fp.ostride == fp.n_comp && fp.ocstride == 1 && fp.ooffset == 0 &&
+               reinterpret_cast<uintptr_t>(dst) % bytes == 0;
+      return f;
+    };
     switch (fp.n_comp)
       {
-#define HPS_CASE(NC)                                                   \
-  case NC:                                                             \
-    q1_fixup<T, NC><<<blocks, 256, 0, s>>>(fp, part, dst);             \
+#define HPS_CASE(NC)                                                               \
+  case NC:                                                                         \
+    q1_fixup<T, NC><<<blocks, 256, 0, s>>>(with_vdst(FixVec<T, NC>::bytes), part, dst); \
     break;
         HPS_CASE(3)
         HPS_CASE(4)
