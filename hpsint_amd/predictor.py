@@ -9,7 +9,7 @@ correction of the linear extrapolation 2 u_n - u_{n-1} on (c, mu, eta_top1,
 eta_top2).  All arithmetic runs in the library's BF16-MFMA kernels (FP32
 accumulation); torch tensors only hold the memory.  The plain Newton solve
 from the predicted guess keeps the same tolerance, so the plain method stays
-both fallback and judge.
+both the fallback and the reference for the result.
 """
 import ctypes
 import math
