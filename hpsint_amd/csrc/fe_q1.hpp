@@ -286,6 +286,25 @@ namespace hps
       }
   }
 
+  // the value of interp_vg_hat alone (same levels in the same order: the
+  // same rounding)
+  template <int DIM, typename T>
+  __device__ __forceinline__ T interp_v_hat(T u)
+  {
+    const T r = Q1Hat<T>::r;
+    if constexpr (DIM == 2)
+      {
+        const T A = u + r * xshfl<1>(u);
+        return A + r * xshfl<2>(A);
+      }
+    else
+      {
+        const T A  = u + r * xshfl<4>(u);
+        const T AA = A + r * xshfl<2>(A);
+        return AA + r * xshfl<1>(AA);
+      }
+  }
+
   // FoldScales of the scalar-mobility tangent for the rescaled butterflies
   // (the scales of KParams, the mobility-derivative one signed by the lane's
   // corner) and the value-row factor jxw_hat
@@ -510,6 +529,12 @@ namespace hps
     // layer's waits do not cover them and both arrive during the node sums
     // and the next interpolation
     constexpr bool PF = FOLD && DIM == 3 && G <= 4;
+    // TWO (FP64, 3D, 5 or more order parameters): values first, each
+    // gradient interpolated again where its row is integrated, so the
+    // gradients of all components are never live together (8 order
+    // parameters: 246 -> 167 VGPRs, 2 -> 3 waves/SIMD; the FP32 kernels and
+    // those with fewer order parameters are faster without it)
+    constexpr bool TWO = FOLD && DIM == 3 && G >= 5 && sizeof(T) == 8;
     T              lin_next[PF ? NF : 1];
     if constexpr (PF)
       {
@@ -531,7 +556,9 @@ namespace hps
         T        val[NC], grad[NC][DIM];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-          if constexpr (FOLD)
+          if constexpr (TWO)
+            val[c] = interp_v_hat<DIM>(pl[c]);
+          else if constexpr (FOLD)
             interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
           else
             interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
@@ -565,11 +592,23 @@ namespace hps
 #pragma unroll
             for (int c = 0; c < NC; ++c)
               {
+                // TWO: the gradient this row needs is interpolated here
+                // (grad mu for the c row, grad c for the mu row)
+                T g[DIM];
+                if constexpr (TWO)
+                  {
+                    T v_unused;
+                    interp_vg_hat<DIM>(pl[c == 0 ? 1 : (c == 1 ? 0 : c)], v_unused, g);
+                  }
+                else
+#pragma unroll
+                  for (int d = 0; d < DIM; ++d)
+                    g[d] = grad[c == 0 ? 1 : (c == 1 ? 0 : c)][d];
                 T gs[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)
-                  gs[d] = c == 0 ? (M * fs.kj[d]) * grad[1][d] + s * (lmu[d] * fs.gsj[d])
-                                 : (c == 1 ? fs.kmu[d] * grad[0][d] : fs.keta[d] * grad[c][d]);
+                  gs[d] = c == 0 ? (M * fs.kj[d]) * g[d] + s * (lmu[d] * fs.gsj[d])
+                                 : (c == 1 ? fs.kmu[d] * g[d] : fs.keta[d] * g[d]);
                 sl[c] = integrate_vg_hat<DIM>(vr[c] * jxw_hat, gs);
               }
           }
