@@ -286,6 +286,25 @@ namespace hps
       }
   }
 
+  // the value of interp_vg alone (same levels in the same order: the same
+  // rounding)
+  template <int DIM, typename T>
+  __device__ __forceinline__ T interp_v_vg(T u)
+  {
+    const T sa = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    if constexpr (DIM == 2)
+      {
+        const T A = sa * u + sb * xshfl<1>(u);
+        return sa * A + sb * xshfl<2>(A);
+      }
+    else
+      {
+        const T A  = sa * u + sb * xshfl<4>(u);
+        const T AA = sa * A + sb * xshfl<2>(A);
+        return sa * AA + sb * xshfl<1>(AA);
+      }
+  }
+
   // the value of interp_vg_hat alone (same levels in the same order: the
   // same rounding)
   template <int DIM, typename T>
@@ -535,6 +554,9 @@ namespace hps
     // parameters: 246 -> 167 VGPRs, 2 -> 3 waves/SIMD; the FP32 kernels and
     // those with fewer order parameters are faster without it)
     constexpr bool TWO = FOLD && DIM == 3 && G >= 5 && sizeof(T) == 8;
+    // the same for the scalar-mobility residual (its gradient rows are
+    // M grad mu, -kappa_c grad c and L kappa_p grad eta_i)
+    constexpr bool TWO_R = !TENS && OP != OP_VMULT && DIM == 3 && G >= 5 && sizeof(T) == 8;
     T              lin_next[PF ? NF : 1];
     if constexpr (PF)
       {
@@ -558,6 +580,8 @@ namespace hps
         for (int c = 0; c < NC; ++c)
           if constexpr (TWO)
             val[c] = interp_v_hat<DIM>(pl[c]);
+          else if constexpr (TWO_R)
+            val[c] = interp_v_vg<DIM>(pl[c]);
           else if constexpr (FOLD)
             interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
           else
@@ -631,7 +655,10 @@ namespace hps
                 for (int c = 0; c < NC; ++c)
                   hist[c] = interp_v<DIM>(ph[c]);
               }
-            residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, val, grad, hist, vr, gr);
+            if constexpr (TWO_R)
+              residual_values<DIM, G, TENS, OP - OP_RES0>(kp.pp, val, hist, vr);
+            else
+              residual_qp<DIM, G, TENS, OP - OP_RES0>(kp.pp, val, grad, hist, vr, gr);
           }
 
         if constexpr (!FOLD)
@@ -639,9 +666,28 @@ namespace hps
           for (int c = 0; c < NC; ++c)
             {
               T gs[DIM];
+              if constexpr (TWO_R)
+                {
+                  // this row's gradient input, interpolated here
+                  T v_unused, g[DIM], row[DIM];
+                  interp_vg<DIM>(pl[c == 0 ? 1 : (c == 1 ? 0 : c)], gsc, v_unused, g);
+                  if (c == 0)
+                    residual_grad_c_scalar<DIM, G>(kp.pp, val, g, row);
+                  else
+                    {
+                      const T f = c == 1 ? -T(kp.pp.kappa_c) : T(kp.pp.L) * T(kp.pp.kappa_p);
 #pragma unroll
-              for (int d = 0; d < DIM; ++d)
-                gs[d] = gr[c][d] * gsj[d];
+                      for (int d = 0; d < DIM; ++d)
+                        row[d] = f * g[d];
+                    }
+#pragma unroll
+                  for (int d = 0; d < DIM; ++d)
+                    gs[d] = row[d] * gsj[d];
+                }
+              else
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  gs[d] = gr[c][d] * gsj[d];
               sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
             }
         __syncthreads();

@@ -100,6 +100,9 @@ def test_vmult_f64_matches_oracle(dim, cells, G, tens):
 RES_CASES = [
     (3, (9, 5, 17), 2, False),
     (3, (19, 13, 37), 3, False),
+    (3, (10, 6, 19), 5, False),
+    (3, (9, 5, 17), 8, False),
+    (3, (7, 5, 18), 12, False),
     (3, (11, 6, 18), 3, True),
     (2, (65, 17), 2, False),
     (2, (70, 19), 2, True),
