@@ -65,7 +65,9 @@ namespace hps
 
   // tangent vmult of the tiles tiles[blockIdx.x], each with exactly GA
   // active order parameters (scalar mobility, the folded form of q1_apply
-  // with the rescaled butterflies)
+  // with the rescaled butterflies).  FP64 with one active order parameter,
+  // the most common tile, is held to 96 registers (5 waves/SIMD; its
+  // natural allocation is 99).
   template <int DIM, int TX, int TY, int GA, typename T>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && GA == 1 ? 5 : 1)))
