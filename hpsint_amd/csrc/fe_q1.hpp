@@ -469,6 +469,14 @@ namespace hps
 
     const TL t(kp);
     const int tid = t.tid;
+    // item loops of the 3D scalar tangent with up to 4 order parameters
+    // (plane loads and stores, node sums): thread ids rotated by a wave per
+    // workgroup, so the wave that takes the items beyond NT differs from
+    // one workgroup to the next (elsewhere the plain order: there the
+    // rotation costs an occupancy step)
+    const int itid = (DIM == 3 && !TENS && OP == OP_VMULT && G <= 4)
+                       ? (tid + 64 * (int)(blockIdx.x & 3)) & (NT - 1)
+                       : tid;
 
     const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
     T       gsc[DIM], gsj[DIM];
@@ -489,7 +497,7 @@ namespace hps
 #pragma unroll
       for (int r = 0; r < ITEMS; ++r)
         {
-          const int i = tid + r * NT;
+          const int i = itid + r * NT;
           if (i < PN * NC)
             {
               const int pn = i / NC, c = i - pn * NC;
@@ -519,7 +527,7 @@ namespace hps
 #pragma unroll
       for (int r = 0; r < ITEMS; ++r)
         {
-          const int i = tid + r * NT;
+          const int i = itid + r * NT;
           if (i < PN * NC)
             {
 #pragma unroll
@@ -712,7 +720,7 @@ namespace hps
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r)
           {
-            const int i = tid + r * NT;
+            const int i = itid + r * NT;
             if (i < PN * NC)
               {
                 const int pn = i / NC, c = i - pn * NC;
@@ -743,7 +751,7 @@ namespace hps
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
       {
-        const int i = tid + r * NT;
+        const int i = itid + r * NT;
         if (i < PN * NC)
           {
             const int pn = i / NC, c = i - pn * NC;
