@@ -518,10 +518,10 @@ extern "C" int hps_set_constrained_nodes(hps_operator op, const int64_t *nodes, 
         return fail(HPS_ERR_INVALID_ARGUMENT, "constrained node listed twice");
     }
   HPS_HIP(hipStreamSynchronize(op->stream));
+  // the cut-off masks and tile groups depend on the linearisation point
+  // only, not on the constraints: they stay
   (void)hipFree(op->constrained);
   (void)hipFree(op->constrained_save);
-  (void)hipFree(op->cut_masks);
-  (void)hipFree(op->cut_tiles);
   op->constrained      = nullptr;
   op->constrained_save = nullptr;
   op->n_constrained    = 0;

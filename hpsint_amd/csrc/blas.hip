@@ -666,11 +666,20 @@ namespace hps
       case 8:                   \
         CALL(8);                \
         break;                  \
+      case 9:                   \
+        CALL(9);                \
+        break;                  \
       case 10:                  \
         CALL(10);               \
         break;                  \
+      case 11:                  \
+        CALL(11);               \
+        break;                  \
       case 12:                  \
         CALL(12);               \
+        break;                  \
+      case 13:                  \
+        CALL(13);               \
         break;                  \
       case 14:                  \
         CALL(14);               \

@@ -1,46 +1,37 @@
-// Instantiations: 2D Q1, scalar and tensorial mobility; the fix-up kernel.
+// The Q1 dispatcher and the fix-up kernel.
 #include "ops.hpp"
 
 namespace hps
 {
-  const OpTable *q1_table_d3_scalar(int n_grains);
+  const OpTable *q1_d3s_a(int n_grains);
+  const OpTable *q1_d3s_b(int n_grains);
+  const OpTable *q1_d3s_c(int n_grains);
+  const OpTable *q1_d3t_a(int n_grains);
+  const OpTable *q1_d3t_b(int n_grains);
+  const OpTable *q1_d3t_c(int n_grains);
+  const OpTable *q1_d2s(int n_grains);
+  const OpTable *q1_d2t(int n_grains);
   const OpTable *q1_table_d3_scalar_ty8(int n_grains);
-  const OpTable *q1_table_d3_tensorial(int n_grains);
 
+  // every grain count 1..12 (EXPAND_OPERATIONS, S/instantiation.h:97-152)
   const OpTable *q1_table_d3(int n_grains, bool tensorial, int ty)
   {
+    if (!tensorial && ty == 8)
+      return q1_table_d3_scalar_ty8(n_grains);
+    const OpTable *t = nullptr;
     if (tensorial)
-      return q1_table_d3_tensorial(n_grains);
-    return ty == 8 ? q1_table_d3_scalar_ty8(n_grains) : q1_table_d3_scalar(n_grains);
+      {
+        if (!(t = q1_d3t_a(n_grains)) && !(t = q1_d3t_b(n_grains)))
+          t = q1_d3t_c(n_grains);
+      }
+    else if (!(t = q1_d3s_a(n_grains)) && !(t = q1_d3s_b(n_grains)))
+      t = q1_d3s_c(n_grains);
+    return t;
   }
 
   const OpTable *q1_table_d2(int n_grains, bool tensorial)
   {
-#define HPS_CASE(G, TENS)                                                  \
-  case G:                                                                  \
-    {                                                                      \
-      static const OpTable t = Q1Launch<2, TX2, TY2, G, TENS>::table();    \
-      return &t;                                                           \
-    }
-    if (!tensorial)
-      switch (n_grains)
-        {
-          HPS_CASE(1, false)
-          HPS_CASE(2, false)
-          HPS_CASE(3, false)
-          HPS_CASE(4, false)
-          default:
-            return nullptr;
-        }
-    switch (n_grains)
-      {
-        HPS_CASE(1, true)
-        HPS_CASE(2, true)
-        HPS_CASE(3, true)
-        default:
-          return nullptr;
-      }
-#undef HPS_CASE
+    return tensorial ? q1_d2t(n_grains) : q1_d2s(n_grains);
   }
 
   template <typename T>
@@ -70,8 +61,11 @@ namespace hps
         HPS_CASE(6)
         HPS_CASE(7)
         HPS_CASE(8)
+        HPS_CASE(9)
         HPS_CASE(10)
+        HPS_CASE(11)
         HPS_CASE(12)
+        HPS_CASE(13)
         HPS_CASE(14)
 #undef HPS_CASE
         default:

@@ -1,15 +1,16 @@
-// Instantiations: 3D Q1, scalar mobility.
+// Instantiations: 3D Q1, scalar mobility, 1-4 order parameters.
 #include "ops.hpp"
 
 namespace hps
 {
-  const OpTable *q1_table_d3_scalar(int n_grains)
+  // nullptr for grain counts outside this unit's range
+  const OpTable *q1_d3s_a(int n_grains)
   {
-#define HPS_CASE(G)                                                   \
-  case G:                                                             \
-    {                                                                 \
+#define HPS_CASE(G)                                                     \
+  case G:                                                               \
+    {                                                                   \
       static const OpTable t = Q1Launch<3, TX3, TY3, G, false>::table(); \
-      return &t;                                                      \
+      return &t;                                                        \
     }
     switch (n_grains)
       {
@@ -17,11 +18,6 @@ namespace hps
         HPS_CASE(2)
         HPS_CASE(3)
         HPS_CASE(4)
-        HPS_CASE(5)
-        HPS_CASE(6)
-        HPS_CASE(8)
-        HPS_CASE(10)
-        HPS_CASE(12)
         default:
           return nullptr;
       }

@@ -1,15 +1,16 @@
-// Instantiations: 3D Q1, tensorial mobility.
+// Instantiations: 3D Q1, tensorial mobility, 1-4 order parameters.
 #include "ops.hpp"
 
 namespace hps
 {
-  const OpTable *q1_table_d3_tensorial(int n_grains)
+  // nullptr for grain counts outside this unit's range
+  const OpTable *q1_d3t_a(int n_grains)
   {
-#define HPS_CASE(G)                                                  \
-  case G:                                                            \
-    {                                                                \
+#define HPS_CASE(G)                                                     \
+  case G:                                                               \
+    {                                                                   \
       static const OpTable t = Q1Launch<3, TX3, TY3, G, true>::table(); \
-      return &t;                                                     \
+      return &t;                                                        \
     }
     switch (n_grains)
       {

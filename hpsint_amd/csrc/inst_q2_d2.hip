@@ -1,44 +1,31 @@
-// Instantiations: 2D Q2, scalar and tensorial mobility; the Q2 dispatcher.
+// The Q2 dispatcher.
 #include "ops.hpp"
 
 namespace hps
 {
-  const Q2Table *q2_table_d3_scalar(int n_grains);
-  const Q2Table *q2_table_d3_tensorial(int n_grains);
+  const Q2Table *q2_d3s_a(int n_grains);
+  const Q2Table *q2_d3s_b(int n_grains);
+  const Q2Table *q2_d3s_c(int n_grains);
+  const Q2Table *q2_d3t_a(int n_grains);
+  const Q2Table *q2_d3t_b(int n_grains);
+  const Q2Table *q2_d3t_c(int n_grains);
+  const Q2Table *q2_d3t_d(int n_grains);
+  const Q2Table *q2_d2s(int n_grains);
+  const Q2Table *q2_d2t(int n_grains);
 
-  static const Q2Table *q2_table_d2(int n_grains, bool tensorial)
-  {
-#define HPS_CASE(G, TENS)                                           \
-  case G:                                                           \
-    {                                                               \
-      static const Q2Table t = Q2Launch<2, G, TENS>::table();       \
-      return &t;                                                    \
-    }
-    if (!tensorial)
-      switch (n_grains)
-        {
-          HPS_CASE(1, false)
-          HPS_CASE(2, false)
-          HPS_CASE(3, false)
-          HPS_CASE(4, false)
-          default:
-            return nullptr;
-        }
-    switch (n_grains)
-      {
-        HPS_CASE(1, true)
-        HPS_CASE(2, true)
-        HPS_CASE(3, true)
-        default:
-          return nullptr;
-      }
-#undef HPS_CASE
-  }
-
+  // every grain count 1..12 (EXPAND_OPERATIONS, S/instantiation.h:97-152)
   const Q2Table *q2_table(int dim, int n_grains, bool tensorial)
   {
     if (dim == 2)
-      return q2_table_d2(n_grains, tensorial);
-    return tensorial ? q2_table_d3_tensorial(n_grains) : q2_table_d3_scalar(n_grains);
+      return tensorial ? q2_d2t(n_grains) : q2_d2s(n_grains);
+    const Q2Table *t = nullptr;
+    if (tensorial)
+      {
+        if (!(t = q2_d3t_a(n_grains)) && !(t = q2_d3t_b(n_grains)) && !(t = q2_d3t_c(n_grains)))
+          t = q2_d3t_d(n_grains);
+      }
+    else if (!(t = q2_d3s_a(n_grains)) && !(t = q2_d3s_b(n_grains)))
+      t = q2_d3s_c(n_grains);
+    return t;
   }
 } // namespace hps

@@ -1,0 +1,34 @@
+// Instantiations: 2D Q2, tensorial mobility, 1-12 order parameters.
+#include "ops.hpp"
+
+namespace hps
+{
+  // nullptr for grain counts outside this unit's range
+  const Q2Table *q2_d2t(int n_grains)
+  {
+#define HPS_CASE(G)                                                     \
+  case G:                                                               \
+    {                                                                   \
+      static const Q2Table t = Q2Launch<2, G, true>::table(); \
+      return &t;                                                        \
+    }
+    switch (n_grains)
+      {
+        HPS_CASE(1)
+        HPS_CASE(2)
+        HPS_CASE(3)
+        HPS_CASE(4)
+        HPS_CASE(5)
+        HPS_CASE(6)
+        HPS_CASE(7)
+        HPS_CASE(8)
+        HPS_CASE(9)
+        HPS_CASE(10)
+        HPS_CASE(11)
+        HPS_CASE(12)
+        default:
+          return nullptr;
+      }
+#undef HPS_CASE
+  }
+} // namespace hps

@@ -1,15 +1,16 @@
-// Instantiations: 3D Q2, scalar mobility.
+// Instantiations: 3D Q2, scalar mobility, 1-4 order parameters.
 #include "ops.hpp"
 
 namespace hps
 {
-  const Q2Table *q2_table_d3_scalar(int n_grains)
+  // nullptr for grain counts outside this unit's range
+  const Q2Table *q2_d3s_a(int n_grains)
   {
-#define HPS_CASE(G)                                                 \
-  case G:                                                           \
-    {                                                               \
-      static const Q2Table t = Q2Launch<3, G, false>::table();      \
-      return &t;                                                    \
+#define HPS_CASE(G)                                                     \
+  case G:                                                               \
+    {                                                                   \
+      static const Q2Table t = Q2Launch<3, G, false>::table(); \
+      return &t;                                                        \
     }
     switch (n_grains)
       {

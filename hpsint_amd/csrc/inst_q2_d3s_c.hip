@@ -1,0 +1,26 @@
+// Instantiations: 3D Q2, scalar mobility, 9-12 order parameters.
+#include "ops.hpp"
+
+namespace hps
+{
+  // nullptr for grain counts outside this unit's range
+  const Q2Table *q2_d3s_c(int n_grains)
+  {
+#define HPS_CASE(G)                                                     \
+  case G:                                                               \
+    {                                                                   \
+      static const Q2Table t = Q2Launch<3, G, false>::table(); \
+      return &t;                                                        \
+    }
+    switch (n_grains)
+      {
+        HPS_CASE(9)
+        HPS_CASE(10)
+        HPS_CASE(11)
+        HPS_CASE(12)
+        default:
+          return nullptr;
+      }
+#undef HPS_CASE
+  }
+} // namespace hps

@@ -1,0 +1,25 @@
+// Instantiations: 3D Q2, tensorial mobility, 5-7 order parameters.
+#include "ops.hpp"
+
+namespace hps
+{
+  // nullptr for grain counts outside this unit's range
+  const Q2Table *q2_d3t_b(int n_grains)
+  {
+#define HPS_CASE(G)                                                     \
+  case G:                                                               \
+    {                                                                   \
+      static const Q2Table t = Q2Launch<3, G, true>::table(); \
+      return &t;                                                        \
+    }
+    switch (n_grains)
+      {
+        HPS_CASE(5)
+        HPS_CASE(6)
+        HPS_CASE(7)
+        default:
+          return nullptr;
+      }
+#undef HPS_CASE
+  }
+} // namespace hps

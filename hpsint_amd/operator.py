@@ -167,6 +167,7 @@ class SinteringOperatorGeneric:
         self._n_nodes, self._n_dofs = nn.value, nd.value
         self._bind_stream()
         self.time_data = None
+        self._order = 0
         self._old = []
 
     def __del__(self):
@@ -219,10 +220,23 @@ class SinteringOperatorGeneric:
         arr = (ctypes.c_double * 3)(*(list(w) + [0.0] * (3 - len(w))))
         check(lib().hps_set_time_weights(self._h, time_data.get_primary_weight(),
                                          time_data.get_algebraic_weight(), arr, len(w)))
+        self._order = len(w)
 
     def set_time_weights(self, w0, w_hist=(), w_alg=1.0):
         arr = (ctypes.c_double * 3)(*(list(w_hist) + [0.0] * (3 - len(w_hist))))
         check(lib().hps_set_time_weights(self._h, w0, w_alg, arr, len(w_hist)))
+        self._order = len(w_hist)
+
+    @property
+    def history_order(self):
+        """number of old solutions the current time weights read"""
+        return self._order
+
+    def check_history(self, old):
+        """raises unless `old` holds a vector for every history weight"""
+        if len(old) < self._order:
+            raise ValueError("%d old solutions given, the time weights need %d"
+                             % (len(old), self._order))
 
     def set_old_solutions(self, old):
         self._old = list(old)
@@ -233,6 +247,8 @@ class SinteringOperatorGeneric:
 
     def evaluate_nonlinear_residual(self, dst, src, with_time_derivative=2, old=None):
         old = self._old if old is None else old
+        if with_time_derivative == 2:
+            self.check_history(old)
         arr = (ctypes.c_void_p * max(1, len(old)))(*[o.data_ptr() for o in old])
         check(lib().hps_residual(self._h, with_time_derivative, _ptr(dst), _ptr(src), arr))
 

@@ -58,7 +58,13 @@ def load_restart(path, stepper):
     stepper.u = torch.from_numpy(vecs["solution"]).to(dev)
     stepper.history = [torch.from_numpy(vecs["history_%d" % i]).to(dev)
                        for i in range(int(meta["n_history"]))]
-    stepper.td.dt = [float(x) for x in vecs["dt"]]
+    dt_saved = [float(x) for x in vecs["dt"]]
+    if len(stepper.history) >= stepper.order:
+        stepper.td.dt = dt_saved
+    else:
+        # too few old solutions for the saved scheme: start again from the
+        # lowest order with the current step (S/driver.h:370-377)
+        stepper.td.dt = [dt_saved[0]] + [0.0] * (stepper.order - 1)
     stepper.td.update_weights()
     stepper.t = float(meta["t"])
     stepper.dt = float(meta["dt_next"])

@@ -58,6 +58,7 @@ class NewtonSolver:
     def solve(self, x, old, r0_ref=-1.0):
         """solves R(x) = 0 in place; returns the statistics.  r0_ref > 0 is
         the reference norm of the relative criterion (default: |R(x_0)|)"""
+        self.op.check_history(old)
         arr = (ctypes.c_void_p * max(1, len(old)))(*[o.data_ptr() for o in old])
         st = _ffi.HpsNewtonStats()
         rc = lib().hps_newton_solve_ref(self._h, ctypes.c_void_p(x.data_ptr()), arr,

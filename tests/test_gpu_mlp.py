@@ -199,3 +199,79 @@ def test_predictor_saves_newton_iterations():
     plain = run(None)
     learned = run(pred)
     assert sum(learned) < sum(plain), (plain, learned)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_features_kernel_matches_numpy(dim):
+    """hps_mlp_features against features_numpy on a packing state: 2D on the
+    two-particle fields, 3D on a sub-box of the 100-sphere packing (12 order
+    parameters); bf16 features within one bf16 rounding, top-2 order
+    parameters exact, local scales to 1e-6"""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import config_fields as cf
+    if dim == 2:
+        pk = packing.two_particles_2d()
+        cells, G = (12, 7), 2
+        pk = packing.Packing(pk.centers, pk.radii, (-8.0, -3.5), (4.0, 3.5), pk.colors)
+        u_cur = packing.initial_fields(pk, cells)
+    else:
+        big = packing.packing_cfg4(seed=0)
+        cells, G = (6, 5, 7), 12
+        pk = cf.sub_packing(big, cf.contact_point(big), 6, 1.0)
+        pk = packing.Packing(pk.centers, pk.radii, pk.lower,
+                             (pk.upper[0], pk.upper[1] - 1.0, pk.upper[2] + 1.0), pk.colors)
+        u_cur = cf.state(pk, cells, G)
+    nc = G + 2
+    mesh = CartesianMesh(cells, pk.lower, pk.upper)
+    op = SinteringOperatorGeneric(mesh, G)
+    rng = np.random.default_rng(11)
+    u_prev = u_cur + 0.01 * rng.standard_normal(u_cur.size) * (np.abs(u_cur) > 1e-3)
+    pred = LearnedPredictor(op, MLP(op), dt_scale=1e3)
+    dp, dc = torch.from_numpy(u_prev).cuda(), torch.from_numpy(u_cur).cuda()
+    X, top2 = pred.features(dp, dc, 1e-3)
+    torch.cuda.synchronize()
+    nn = [n + 1 for n in cells] + [1] * (3 - dim)
+    Xr, topr, lsr = features_numpy(u_prev, u_cur, nn, nc, 1e-3, 1e3, pred.inc_scale)
+    np.testing.assert_array_equal(top2.cpu().numpy(), topr)
+    np.testing.assert_allclose(pred.lscale.cpu().numpy(), lsr, rtol=1e-6)
+    got = as64(X)
+    want = mo.bf16(Xr)
+    ulp = np.maximum(np.abs(want), 1e-30) * 2.0 ** -7
+    assert np.all(np.abs(got - want) <= ulp), np.abs(got - want).max()
+
+
+def test_predictor_saves_newton_iterations_3d():
+    """3D: a 40^3 sub-box of the 100-sphere packing (12 order parameters,
+    cell size 1): 12 plain steps train the network, the next 4 steps are
+    solved from both initial guesses at Newton rtol 1e-8 (relative to the
+    plain method's initial residual in both runs)"""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import config_fields as cf
+    from hpsint_amd.predictor_training import collect_trajectory, fit_predictor
+    from hpsint_amd.solvers import NewtonSolver, TimeStepper
+    big = packing.packing_cfg4(seed=0)
+    n, G, dt = 40, 12, 1e-3
+    pk = cf.sub_packing(big, cf.contact_point(big), n, 1.0)
+    mesh = CartesianMesh((n, n, n), pk.lower, pk.upper)
+    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(), MobilityParams())
+    u0 = torch.from_numpy(packing.initial_fields(pk, (n, n, n), n_order_params=G)).cuda()
+    traj = collect_trajectory(op, u0, n_steps=16, dt=dt, nl_rel_tol=1e-8)
+    pred = fit_predictor(op, traj[:13], dt=dt, iterations=1500,
+                         rows_per_step=min(32768, op.n_nodes), seed=0,
+                         column_weights=(3.0, 1.0, 1.0, 1.0))
+
+    def run(predictor):
+        solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2)
+        ts = TimeStepper(op, solver, order=1, dt=dt, predictor=predictor)
+        ts.initialize(traj[12])
+        ts.history = [traj[12].clone(), traj[11].clone()]
+        return [ts.make_step().newton_iterations for _ in range(4)]
+
+    plain = run(None)
+    learned = run(pred)
+    print("3D Newton iterations plain", plain, "learned", learned)
+    assert sum(learned) < sum(plain), (plain, learned)

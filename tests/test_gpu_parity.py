@@ -75,6 +75,17 @@ VMULT_CASES = [
     (2, (65, 17), 2, False),
     (2, (130, 33), 4, False),
     (2, (70, 19), 2, True),
+    # grain counts the benchmark configs do not use (every G of
+    # EXPAND_OPERATIONS, S/instantiation.h:97-152, is built)
+    (3, (9, 5, 17), 7, False),
+    (3, (10, 6, 19), 9, False),
+    (3, (7, 5, 18), 11, False),
+    (3, (9, 5, 11), 5, True),
+    (3, (7, 5, 9), 8, True),
+    (3, (5, 4, 6), 12, True),
+    (2, (65, 17), 7, False),
+    (2, (70, 19), 6, True),
+    (2, (33, 9), 12, True),
 ]
 
 
@@ -106,6 +117,11 @@ RES_CASES = [
     (3, (11, 6, 18), 3, True),
     (2, (65, 17), 2, False),
     (2, (70, 19), 2, True),
+    (3, (10, 6, 19), 9, False),
+    (3, (7, 5, 18), 11, False),
+    (3, (9, 5, 11), 5, True),
+    (3, (7, 5, 9), 8, True),
+    (2, (70, 19), 6, True),
 ]
 
 
@@ -502,3 +518,20 @@ def test_grain_cut_off_vmult_matches_oracle(G, tol):
     op.vmult(dst, dev(src))
     full = po.vmult(p, po.fill_qp(p, u.ravel()), src)
     assert rel_l2(host(dst), full) <= TOL64
+
+
+def test_every_grain_count_is_built():
+    """hps_create accepts every grain count 1..12 for Q1 and Q2, 2D and 3D,
+    scalar and tensorial mobility (EXPAND_OPERATIONS instantiates all of
+    them, S/instantiation.h:97-152); 13 order parameters are refused with
+    ExcInvalidNumberOfComponents"""
+    for dim in (2, 3):
+        mesh = CartesianMesh((3,) * dim, (0.0,) * dim, (3.0,) * dim)
+        for deg in (1, 2):
+            for tens in (False, True):
+                for G in range(1, 13):
+                    SinteringOperatorGeneric(mesh, G, FreeEnergyParams(),
+                                             MobilityParams(tensorial=tens), fe_degree=deg)
+                with pytest.raises(_ffi.InvalidNumberOfComponents):
+                    SinteringOperatorGeneric(mesh, 13, FreeEnergyParams(),
+                                             MobilityParams(tensorial=tens), fe_degree=deg)

@@ -68,6 +68,13 @@ VMULT_CASES = [
     (2, (9, 6), 2, False),
     (2, (1, 5), 4, False),
     (2, (7, 8), 3, True),
+    # grain counts the benchmark configs do not use
+    (3, (3, 4, 2), 6, False),
+    (3, (2, 3, 3), 11, False),
+    (3, (3, 2, 3), 7, True),
+    (3, (2, 2, 3), 12, True),
+    (2, (7, 8), 9, False),
+    (2, (5, 6), 12, True),
 ]
 
 

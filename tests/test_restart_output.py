@@ -66,3 +66,64 @@ def test_vtk_reads_back(tmp_path, cells):
     np.testing.assert_allclose(f["bnds"], (e ** 2).sum(axis=1), rtol=1e-15)
     gb = sum(e[:, i] * e[:, j] for i in range(nc - 2) for j in range(i))
     np.testing.assert_allclose(f["gb"], gb, rtol=1e-14)
+
+
+def test_restart_partial_history_starts_from_bdf1(tmp_path):
+    """a file with the solution only (full_history=False) cannot feed a BDF2
+    step: the integrator restarts from the lowest order with the current
+    step size (S/driver.h:370-377), and history_weights() then asks for
+    exactly the one old solution the file holds"""
+    mesh = CartesianMesh((4, 3, 2), (0.0, 0.0, 0.0), (4.0, 3.0, 2.0))
+    a = fake_stepper(mesh, 5, seed=1)
+    path = str(tmp_path / "p.safetensors")
+    save_restart(path, a, full_history=False)
+    b = fake_stepper(mesh, 5, seed=2)
+    load_restart(path, b)
+    assert len(b.history) == 1
+    assert b.td.dt == [a.td.dt[0], 0.0]
+    assert b.td.effective_order() == 1
+    assert len(b.td.history_weights()) == 1
+    dt = a.td.dt[0]
+    assert b.td.weights[:2] == [1.0 / dt, -1.0 / dt]
+
+
+def test_bdf_weights_closed_forms():
+    """TimeIntegratorData / BDF{1,2,3}Scheme::compute_weights
+    (I/time_integration/time_schemes.h:65-148): constant steps give the
+    textbook BDF coefficients; variable steps differentiate every polynomial
+    of degree <= order exactly at the new time level"""
+    dt = 0.01
+    td = TimeIntegratorData(order=1, dt=dt)
+    np.testing.assert_allclose(td.weights, [1 / dt, -1 / dt], rtol=1e-15)
+    td = TimeIntegratorData(order=2, dt=dt)
+    td.update_dt(dt)
+    np.testing.assert_allclose(td.weights, np.array([1.5, -2.0, 0.5]) / dt, rtol=1e-13)
+    td = TimeIntegratorData(order=3, dt=dt)
+    td.update_dt(dt)
+    assert td.effective_order() == 2           # two valid steps: BDF2 weights
+    np.testing.assert_allclose(td.weights[:3], np.array([1.5, -2.0, 0.5]) / dt, rtol=1e-13)
+    assert len(td.history_weights()) == 2
+    td.update_dt(dt)
+    np.testing.assert_allclose(td.weights, np.array([11 / 6, -3.0, 1.5, -1 / 3]) / dt,
+                               rtol=1e-13)
+    # variable steps: sum_k w_k p(t_{n+1-k}) = p'(t_{n+1}) for deg p <= order
+    rng = np.random.default_rng(5)
+    for order in (1, 2, 3):
+        steps = rng.uniform(0.5e-3, 2e-3, size=order)
+        td = TimeIntegratorData(order=order, dt=steps[-1])
+        for s in steps[-2::-1]:
+            td.update_dt(s)
+        # td.dt[0] is the newest step; times t_{n+1}, t_n, ... going back
+        times = [0.0]
+        for s in td.dt[:order]:
+            times.append(times[-1] - s)
+        for deg in range(order + 1):
+            coef = rng.standard_normal(deg + 1)
+            p = np.polynomial.Polynomial(coef)
+            lhs = sum(w * p(t) for w, t in zip(td.weights, times))
+            assert abs(lhs - p.deriv()(0.0)) <= 1e-9 * (1 + abs(p.deriv()(0.0))), \
+                (order, deg, lhs, p.deriv()(0.0))
+    # an order-3 scheme with one valid step falls back to BDF1
+    td = TimeIntegratorData(order=3, dt=dt)
+    assert td.effective_order() == 1
+    np.testing.assert_allclose(td.weights[:2], [1 / dt, -1 / dt], rtol=1e-15)
