@@ -114,6 +114,8 @@ _SIGNATURES = {
     "hps_residual": [vp, cint, vp, vp, ctypes.POINTER(vp)],
     "hps_set_constrained_nodes": [vp, ctypes.POINTER(i64), i64],
     "hps_set_grain_cut_off": [vp, dbl],
+    "hps_set_component_mask": [vp, vp],
+    "hps_get_component_mask": [vp, vp],
     "hps_vmult_f64": [vp, vp, vp],
     "hps_vmult_f32": [vp, vp, vp],
     "hps_precond_update": [vp, cint],

@@ -85,9 +85,11 @@ struct hps_operator_s
   hipEvent_t           ev[3];
   bool                 timing;
   const double        *old[3];
-  double               cut_tol;          // grain cut-off tolerance (< 0: off)
-  bool                 cut_ready;        // tile groups of the current linearisation
-  unsigned            *cut_masks;        // per tile: bit g = eta_g active
+  double               cut_tol;          // grain cut-off tolerance (<= 0: off)
+  bool                 cut_active;       // a component table is set (cell_masks)
+  unsigned            *cell_masks;       // per cell: bit g = eta_g active
+  bool                 cut_ready;        // tile groups for the fast cut vmult
+  unsigned            *cut_masks;        // per tile: union of its cell masks
   int                 *cut_tiles;        // tiles grouped by active count
   int                  cut_off[hps::CUT_MAX_GA + 2];
   int64_t              n_constrained;    // constrained nodes (none by default)
@@ -313,6 +315,7 @@ extern "C" int hps_destroy(hps_operator op)
   (void)hipFree(op->constrained_save);
   (void)hipFree(op->cut_masks);
   (void)hipFree(op->cut_tiles);
+  (void)hipFree(op->cell_masks);
   for (int i = 0; i < 3; ++i)
     if (op->ev[i])
       (void)hipEventDestroy(op->ev[i]);
@@ -362,18 +365,22 @@ extern "C" int hps_set_time_weights(hps_operator op, double w0, double w_alg,
   return HPS_OK;
 }
 
-// per-tile active order parameters of the linearisation point u, tiles
-// grouped by their number of active order parameters
-static int compute_cut(hps_operator op, const double *u)
+// the fast cut vmult (3D, scalar mobility, default tiles): tile masks as
+// the union of their cells' masks, tiles grouped by their number of active
+// order parameters; a tile with more than CUT_MAX_GA leaves the masked full
+// kernels in use
+static int group_cut_tiles(hps_operator op)
 {
   op->cut_ready = false;
-  const int nt  = op->n_tiles;
+  if (op->dim != 3 || op->tens || op->ops->ty != hps::TY3 || op->n_grains > 32)
+    return HPS_OK;
+  const int nt = op->n_tiles;
   if (!op->cut_masks)
     {
       HPS_HIP(hipMalloc(&op->cut_masks, nt * sizeof(unsigned)));
       HPS_HIP(hipMalloc(&op->cut_tiles, nt * sizeof(int)));
     }
-  hps::tile_masks_d3(op->kp, nt, op->n_grains, op->cut_tol, u, op->cut_masks, op->stream);
+  hps::tile_masks_d3(op->kp, nt, op->cut_masks, op->stream);
   HPS_LAUNCHED();
   std::vector<unsigned> m(nt);
   HPS_HIP(hipMemcpyAsync(m.data(), op->cut_masks, nt * sizeof(unsigned), hipMemcpyDeviceToHost,
@@ -384,7 +391,7 @@ static int compute_cut(hps_operator op, const double *u)
     {
       const int ga = __builtin_popcount(m[i]);
       if (ga > hps::CUT_MAX_GA)
-        return HPS_OK; // a tile with more active order parameters: full vmult
+        return HPS_OK;
       ++count[ga];
     }
   op->cut_off[0] = 0;
@@ -401,21 +408,63 @@ static int compute_cut(hps_operator op, const double *u)
   return HPS_OK;
 }
 
+// drops the component table; a cache filled with it is no longer valid
+static void clear_cut(hps_operator op)
+{
+  if (op->cut_active)
+    op->lin_set = false;
+  op->cut_active = false;
+  op->cut_ready  = false;
+  op->kp.cmask   = nullptr;
+}
+
 extern "C" int hps_set_grain_cut_off(hps_operator op, double tol)
 {
   if (int r = check_op(op))
     return r;
-  if (tol < 0.0)
+  clear_cut(op);
+  if (!(tol > 0.0))
     {
-      op->cut_tol   = -1.0;
-      op->cut_ready = false;
+      op->cut_tol = -1.0;
       return HPS_OK;
     }
-  if (op->q2 || op->dim != 3 || op->tens || op->ops->ty != hps::TY3 || op->n_grains > 32)
-    return fail(HPS_ERR_INVALID_ARGUMENT,
-                "grain cut-off is built for Q1, 3D, scalar mobility, default tiles");
-  op->cut_tol   = tol;
-  op->cut_ready = false; // the tile groups follow at the next linearisation
+  if (op->q2)
+    return fail(HPS_ERR_UNSUPPORTED, "grain cut-off is built for Q1 operators");
+  if (op->n_grains > 32)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "grain cut-off: at most 32 order parameters");
+  op->cut_tol = tol;
+  return HPS_OK;
+}
+
+extern "C" int hps_set_component_mask(hps_operator op, const double *u)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!u)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "null vector");
+  if (!(op->cut_tol > 0.0))
+    return fail(HPS_ERR_NOT_INITIALIZED, "component mask without a grain cut-off tolerance");
+  const int64_t n_cells = (int64_t)op->kp.nc[0] * op->kp.nc[1] * op->kp.nc[2];
+  if (!op->cell_masks)
+    HPS_HIP(hipMalloc(&op->cell_masks, n_cells * sizeof(unsigned)));
+  clear_cut(op);
+  op->lin_set = false; // the cache is filled with the new table
+  hps::cell_masks(op->kp, op->dim, op->n_grains, op->cut_tol, u, op->cell_masks, op->stream);
+  HPS_LAUNCHED();
+  op->kp.cmask    = op->cell_masks;
+  op->cut_active  = true;
+  return group_cut_tiles(op);
+}
+
+extern "C" int hps_get_component_mask(hps_operator op, uint32_t *masks)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!op->cut_active)
+    return fail(HPS_ERR_NOT_INITIALIZED, "no component table set");
+  const int64_t n_cells = (int64_t)op->kp.nc[0] * op->kp.nc[1] * op->kp.nc[2];
+  HPS_HIP(hipMemcpyAsync(masks, op->cell_masks, n_cells * sizeof(unsigned),
+                         hipMemcpyDeviceToDevice, op->stream));
   return HPS_OK;
 }
 
@@ -432,12 +481,12 @@ extern "C" int hps_set_linearization(hps_operator op, const double *u)
     }
   if (op->q2)
     op->q2->fill(q2_params(op), u, op->cache64, op->cache32, op->stream);
+  else if (op->cut_active)
+    op->ops->fill_cut(op->kp, op->n_tiles, u, op->cache64, op->cache32, op->stream);
   else
     op->ops->fill(op->kp, op->n_tiles, u, op->cache64, op->cache32, op->stream);
   HPS_LAUNCHED();
   op->lin_set = true;
-  if (op->cut_tol >= 0.0)
-    return compute_cut(op, u);
   return HPS_OK;
 }
 
@@ -483,8 +532,8 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
   if (op->q2)
     op->q2->residual(q2_params(op), w, src, o[0], o[1], o[2], dst, op->stream);
   else
-    op->ops->residual(op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part,
-                      op->stream);
+    (op->cut_active ? op->ops->residual_cut : op->ops->residual)(
+      op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[1], op->stream));
@@ -563,7 +612,9 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
                             op->cut_masks, src, op->cache64, dst, op->part, op->stream);
       }
   else
-    op->ops->vmult64(op->kp, op->n_tiles, src, dst, op->part, op->cache64, op->stream);
+    (op->cut_active ? op->ops->vmult64_cut : op->ops->vmult64)(op->kp, op->n_tiles, src, dst,
+                                                               op->part, op->cache64,
+                                                               op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[1], op->stream));
@@ -609,8 +660,8 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
                             op->stream);
       }
   else
-    op->ops->vmult32(op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32,
-                     op->stream);
+    (op->cut_active ? op->ops->vmult32_cut : op->ops->vmult32)(
+      op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(op->ev[1], op->stream));
@@ -650,11 +701,12 @@ static int precond_update(hps_operator op, int kind, bool fp32)
       else if (op->q2)
         op->q2->probe(q2_params(op), op->cache64, op->diag, -1, nc, 1, 0, op->stream);
       else if (fp32)
-        op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_diag32, (float *)op->part,
-                         -1, nc, 1, 0, op->stream);
+        (op->cut_active ? op->ops->probe32_cut : op->ops->probe32)(
+          op->kp, op->n_tiles, op->cache32, op->inv_diag32, (float *)op->part, -1, nc, 1, 0,
+          op->stream);
       else
-        op->ops->probe(op->kp, op->n_tiles, op->cache64, op->diag, op->part, -1, nc, 1, 0,
-                       op->stream);
+        (op->cut_active ? op->ops->probe_cut : op->ops->probe)(
+          op->kp, op->n_tiles, op->cache64, op->diag, op->part, -1, nc, 1, 0, op->stream);
       HPS_LAUNCHED();
       if (fp32)
         {
@@ -693,11 +745,13 @@ static int precond_update(hps_operator op, int kind, bool fp32)
       for (int j = 0; j < nc && !op->q2; ++j)
         {
           if (fp32)
-            op->ops->probe32(op->kp, op->n_tiles, op->cache32, op->inv_blocks32,
-                             (float *)op->part, j, nc * nc, 1, j * nc, op->stream);
+            (op->cut_active ? op->ops->probe32_cut : op->ops->probe32)(
+              op->kp, op->n_tiles, op->cache32, op->inv_blocks32, (float *)op->part, j,
+              nc * nc, 1, j * nc, op->stream);
           else
-            op->ops->probe(op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc,
-                           1, j * nc, op->stream);
+            (op->cut_active ? op->ops->probe_cut : op->ops->probe)(
+              op->kp, op->n_tiles, op->cache64, op->blocks, op->part, j, nc * nc, 1, j * nc,
+              op->stream);
           HPS_LAUNCHED();
           if (fp32)
             {
@@ -709,6 +763,9 @@ static int precond_update(hps_operator op, int kind, bool fp32)
         }
       if (fp32)
         hps::blas_f32_to_f64(op->stream, nn * nc * nc, op->blocks, op->inv_blocks32);
+      // cut-off: unit diagonal for the components without any entry
+      if (op->cut_active)
+        hps::unit_empty_rows(op->stream, nn, nc, op->blocks);
       // identity rows and columns of the constrained nodes: identity blocks
       if (op->n_constrained)
         hps::constraint_identity_blocks(op->stream, op->n_constrained, nc, op->constrained,

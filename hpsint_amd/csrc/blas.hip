@@ -688,6 +688,32 @@ namespace hps
         return false;           \
     }
 
+  // grain cut-off: a component with no entry in its node block's row and
+  // column (an order parameter dropped in every cell around the node) gets a
+  // unit diagonal, the rule of the point-Jacobi diagonal (|d| <= 1e-10 -> 1,
+  // S/operator_base.h:364-389); blocks stored column by column per node
+  __global__ void k_unit_empty_rows(int64_t n_nodes, int nc, double *__restrict__ blocks)
+  {
+    for (int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; node < n_nodes;
+         node += (int64_t)gridDim.x * blockDim.x)
+      {
+        double *b = blocks + node * nc * nc;
+        for (int a = 2; a < nc; ++a)
+          {
+            bool empty = true;
+            for (int j = 0; j < nc && empty; ++j)
+              empty = b[j * nc + a] == 0.0 && b[a * nc + j] == 0.0;
+            if (empty)
+              b[a * nc + a] = 1.0;
+          }
+      }
+  }
+
+  void unit_empty_rows(hipStream_t s, int64_t n_nodes, int nc, double *blocks)
+  {
+    k_unit_empty_rows<<<grid_for(n_nodes, 256), 256, 0, s>>>(n_nodes, nc, blocks);
+  }
+
   bool invert_blocks(hipStream_t s, int64_t n_nodes, int nc, const double *blocks,
                      double *inv64, float *inv32, int *singular)
   {

@@ -28,6 +28,7 @@ namespace hps
   void invert_diag(hipStream_t, int64_t n, const double *d, double *inv64, float *inv32);
   void diag_apply64(hipStream_t, int64_t n, const double *inv, const double *src, double *dst);
   void diag_apply32(hipStream_t, int64_t n, const float *inv, const float *src, float *dst);
+  void unit_empty_rows(hipStream_t, int64_t n_nodes, int nc, double *blocks);
   bool invert_blocks(hipStream_t, int64_t n_nodes, int nc, const double *blocks, double *inv64,
                      float *inv32, int *singular);
   bool block_apply64(hipStream_t, int64_t n_nodes, int nc, const double *inv, const double *src,

@@ -56,6 +56,9 @@ namespace hps
     // JxW/h^2 sa^(2 DIM - 2), times -kappa_c, times L kappa_p;
     // JxW/h sa^(2 DIM - 1) (signed per lane by the kernels); JxW sa^(2 DIM)
     double fold_kj[3], fold_kmu[3], fold_keta[3], fold_gs[3], fold_jxw;
+    // grain cut-off: active order parameters per cell (bit g: eta_g), cells
+    // x fastest; read only by the CUT kernels
+    const unsigned *cmask;
   };
 
   // fills the uniform folded-tangent scales of kp from jxw, inv_h and pp
@@ -398,6 +401,22 @@ namespace hps
       pn_lane    = (cx + bx) + PX * (cy + by);
     }
 
+    // the lane's cell in layer k (cells x fastest, as the cut-off masks)
+    __device__ __forceinline__ int64_t cell_index(const KParams &kp, int k) const
+    {
+      return ((int64_t)(m0 + k) * kp.nc[1] + (y0 + cy)) * kp.nc[0] + (x0 + cx);
+    }
+
+    // the cut-off mask of the lane's cell in layer k (all ones without CUT)
+    template <bool CUT>
+    __device__ __forceinline__ unsigned cut_mask(const KParams &kp, int k) const
+    {
+      if constexpr (CUT)
+        return cell_valid ? kp.cmask[cell_index(kp, k)] : 0u;
+      else
+        return ~0u;
+    }
+
     __device__ __forceinline__ bool node_in_tile(int nx, int ny) const
     {
       return nx <= tx_eff && (DIM == 2 || ny <= ty_eff);
@@ -446,7 +465,13 @@ namespace hps
     return (sizeof(T) == 8 && !TENS && OP == OP_VMULT && G <= 4) ? 3 : 1;
   }
 
-  template <int DIM, int TX, int TY, int G, bool TENS, int OP, typename T>
+  // CUT: grain cut-off (do_vmult_range_no_template and
+  // do_evaluate_nonlinear_residual_nt, S/operator_sintering_generic.h:
+  // 456-537, 1008-1081): in every cell the order parameters outside the
+  // cell's mask are dropped -- their values and gradients do not enter the
+  // physics (the cut cache holds zeros for their linearisation values) and
+  // their rows get nothing from the cell
+  template <int DIM, int TX, int TY, int G, bool TENS, int OP, typename T, bool CUT = false>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
     __attribute__((amdgpu_waves_per_eu(q1_apply_waves<G, TENS, OP, T>())))
     q1_apply(const KParams kp, const T *__restrict__ src, const T *__restrict__ old0,
@@ -594,6 +619,20 @@ namespace hps
             interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
           else
             interp_vg<DIM>(pl[c], gsc, val[c], grad[c]);
+        // cut-off: the dropped order parameters of the lane's cell enter
+        // the physics as absent (zero)
+        const unsigned cm = t.template cut_mask<CUT>(kp, k);
+        if constexpr (CUT)
+#pragma unroll
+          for (int c = 2; c < NC; ++c)
+            {
+              const bool on = (cm >> (c - 2)) & 1u;
+              val[c]        = on ? val[c] : T(0);
+              if constexpr (!TWO && !TWO_R)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  grad[c][d] = on ? grad[c][d] : T(0);
+            }
         // the physics stays below the interpolation (with PF it waits for
         // lin_next; in 3D it also shortens the register live ranges)
         if constexpr (FOLD && DIM == 3)
@@ -642,6 +681,9 @@ namespace hps
                   gs[d] = c == 0 ? (M * fs.kj[d]) * g[d] + s * (lmu[d] * fs.gsj[d])
                                  : (c == 1 ? fs.kmu[d] * g[d] : fs.keta[d] * g[d]);
                 sl[c] = integrate_vg_hat<DIM>(vr[c] * jxw_hat, gs);
+                if constexpr (CUT)
+                  if (c >= 2 && !((cm >> (c - 2)) & 1u))
+                    sl[c] = T(0);
               }
           }
         else if constexpr (OP == OP_VMULT)
@@ -697,6 +739,9 @@ namespace hps
                 for (int d = 0; d < DIM; ++d)
                   gs[d] = gr[c][d] * gsj[d];
               sl[c] = integrate_vg<DIM>(vr[c] * jxw, gs);
+              if constexpr (CUT)
+                if (c >= 2 && !((cm >> (c - 2)) & 1u))
+                  sl[c] = T(0);
             }
         __syncthreads();
 
@@ -763,7 +808,10 @@ namespace hps
   // ------------------------------------------- linearisation cache fill
   // SinteringOperatorData::fill_quadrature_point_values,
   // S/operator_sintering_data.h:301-409 (only the fields the tangent reads)
-  template <int DIM, int TX, int TY, int G, bool TENS>
+  // CUT: the linearisation values (and, tensorial, gradients) of the order
+  // parameters dropped in a cell are stored as zeros, as the compacted cache
+  // of the reference leaves them out (S/operator_sintering_data.h:360-395)
+  template <int DIM, int TX, int TY, int G, bool TENS, bool CUT = false>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
     q1_fill(const KParams kp, const double *__restrict__ lin, double *__restrict__ cache64,
             float *__restrict__ cache32)
@@ -820,14 +868,19 @@ namespace hps
         if (pre)
           load_plane(k + 2);
 
-        const double *pl = &ring[(k + t.bm) & 1][t.pn_lane * NC];
-        double        f[NF];
+        const double  *pl = &ring[(k + t.bm) & 1][t.pn_lane * NC];
+        const unsigned cm = t.template cut_mask<CUT>(kp, k);
+        double         f[NF];
         if constexpr (!TENS)
           {
             f[CL::c] = interp_v<DIM>(pl[0]);
 #pragma unroll
             for (int i = 0; i < G; ++i)
-              f[CL::eta + i] = interp_v<DIM>(pl[2 + i]);
+              {
+                f[CL::eta + i] = interp_v<DIM>(pl[2 + i]);
+                if constexpr (CUT)
+                  f[CL::eta + i] = ((cm >> i) & 1u) ? f[CL::eta + i] : 0.0;
+              }
             double v, g[DIM];
             interp_vg<DIM>(pl[1], gsc, v, g);
 #pragma unroll
@@ -852,6 +905,14 @@ namespace hps
               {
                 interp_vg<DIM>(pl[2 + i], gsc, v, leg[i]);
                 f[CL::eta + i] = v;
+                if constexpr (CUT)
+                  if (!((cm >> i) & 1u))
+                    {
+                      f[CL::eta + i] = 0.0;
+#pragma unroll
+                      for (int d = 0; d < DIM; ++d)
+                        leg[i][d] = 0.0;
+                    }
               }
             gb_fields<DIM, G>(f + CL::eta, leg, f + CL::grad_mu, f[CL::gb_s], f + CL::gb_p,
                               f + CL::gb_q);
@@ -1030,7 +1091,9 @@ namespace hps
       }
   }
 
-  template <int DIM, int TX, int TY, int G, bool TENS, typename T>
+  // CUT: entries of rows or columns of order parameters dropped in a cell
+  // get nothing from that cell
+  template <int DIM, int TX, int TY, int G, bool TENS, typename T, bool CUT = false>
   __global__ void __launch_bounds__(TX *TY *(1 << DIM))
     q1_probe(const KParams kp, const T *__restrict__ cache, T *__restrict__ dst,
              T *__restrict__ part, int column, int ostride, int ocstride, int ooffset)
@@ -1065,6 +1128,17 @@ namespace hps
 
         T mine[NC];
         q1_probe_cell<DIM, G, TENS>(kp, t.corner, lin, column, inv_h, mine);
+        if constexpr (CUT)
+          {
+            const unsigned cm   = t.template cut_mask<CUT>(kp, k);
+            const bool     coff = column >= 2 && !((cm >> (column - 2)) & 1u);
+#pragma unroll
+            for (int c = 2; c < NC; ++c)
+              if (coff || !((cm >> (c - 2)) & 1u))
+                mine[c] = T(0);
+            if (coff)
+              mine[0] = mine[1] = T(0);
+          }
         T *sl = &slots[t.corner][t.pn_lane * NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c)

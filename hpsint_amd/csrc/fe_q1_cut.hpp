@@ -1,16 +1,20 @@
-// fe_q1_cut.hpp -- grain cut-off for the Q1 tangent vmult (scalar mobility).
+// fe_q1_cut.hpp -- grain cut-off: per-cell masks and the fast tangent vmult
+// of the 3D scalar-mobility Q1 operator.
 //
 // The reference's cut-off (SinteringOperatorData::set_component_mask,
-// S/operator_sintering_data.h:150-298, and do_vmult_range_no_template,
-// S/operator_sintering_generic.h:456-537) evaluates each cell batch with
-// only the order parameters whose nodal values exceed the tolerance
-// somewhere in the batch: the others are dropped from the batch's physics
-// and receive no contribution from it.  Here the batch is a tile (the
-// TX x TY x tz cells one workgroup marches through): k_tile_masks sets bit
-// g of the tile's mask when |eta_g| > tol at some node of the tile, the
-// host groups the tiles by their number of active order parameters GA, and
-// q1_vmult_cut<GA> runs the tangent of a tile with GA + 2 components (c,
-// mu and the active eta), writing zeros to the inactive components of the
+// S/operator_sintering_data.h:150-298, with do_vmult_range_no_template and
+// do_evaluate_nonlinear_residual_nt, S/operator_sintering_generic.h:456-537,
+// 1008-1081) evaluates each cell batch with only the order parameters whose
+// nodal values exceed the tolerance somewhere in the batch: the others are
+// dropped from the batch's physics and receive no contribution from it.
+// Here the batch is one cell (the reference with a batch width of one):
+// k_cell_masks sets bit g of a cell's mask when max |eta_g| over the cell's
+// nodes > tol.  The masked kernels (CUT = true in fe_q1.hpp) apply the
+// masks to all G order parameters.  The fast tangent vmult below runs a tile
+// (the TX x TY x tz cells of one workgroup) with GA + 2 components: c, mu and
+// the GA order parameters active in some cell of the tile (the union of its
+// cell masks, k_tile_masks); inside the tile every cell still drops those
+// of its own mask's zeros.  It writes zeros to the other components of the
 // tile's nodes, so dst and the shared-node partials are complete for the
 // usual fix-up.
 #pragma once
@@ -19,32 +23,50 @@
 
 namespace hps
 {
-  // bit g: |eta_g| > tol at some node of the tile (nodes of its cells)
-  template <int DIM, int TX, int TY>
-  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
-    k_tile_masks(const KParams kp, int G, double tol, const double *__restrict__ u,
-                 unsigned *__restrict__ masks)
+  // bit g of masks[cell]: max |eta_g| over the cell's nodes > tol
+  // (SinteringOperatorData::set_component_mask's l-infinity rule); one
+  // thread per cell, cells x fastest
+  __global__ void __launch_bounds__(256)
+    k_cell_masks(int dim, int64_t n_cells, int ncx, int ncy, int nnx, int nny, int G,
+                 double tol, const double *__restrict__ u, unsigned *__restrict__ masks)
   {
-    using TL = Tile<DIM, TX, TY>;
-    const TL        t(kp);
-    const int       NC = G + 2;
-    __shared__ unsigned m_sh;
-    if (t.tid == 0)
-      m_sh = 0u;
-    __syncthreads();
-    unsigned  m     = 0u;
-    const int total = TL::PN * (t.tm_eff + 1);
-    for (int i = t.tid; i < total; i += TL::NT)
+    const int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (cell >= n_cells)
+      return;
+    const int     cx = (int)(cell % ncx);
+    const int64_t r  = cell / ncx;
+    const int     cy = (int)(r % ncy);
+    const int     cz = (int)(r / ncy);
+    const int     NC = G + 2;
+    const int     nv = dim == 3 ? 8 : 4;
+    unsigned      m  = 0u;
+    for (int v = 0; v < nv; ++v)
       {
-        const int k = i / TL::PN, pn = i - k * TL::PN;
-        const int nx = pn % TL::PX, ny = pn / TL::PX;
-        if (!t.node_in_tile(nx, ny))
-          continue;
-        const double *x = u + t.gnode(kp, nx, ny, k) * NC + 2;
+        const int64_t node = ((int64_t)(cz + (v >> 2)) * nny + (cy + ((v >> 1) & 1))) * nnx +
+                             (cx + (v & 1));
+        const double *x = u + node * NC + 2;
         for (int g = 0; g < G; ++g)
           if (fabs(x[g]) > tol)
             m |= 1u << g;
       }
+    masks[cell] = m;
+  }
+
+  // union of the cell masks of each tile
+  template <int DIM, int TX, int TY>
+  __global__ void __launch_bounds__(TX *TY *(1 << DIM))
+    k_tile_masks(const KParams kp, unsigned *__restrict__ masks)
+  {
+    using TL = Tile<DIM, TX, TY>;
+    const TL        t(kp);
+    __shared__ unsigned m_sh;
+    if (t.tid == 0)
+      m_sh = 0u;
+    __syncthreads();
+    unsigned m = 0u;
+    if (t.corner == 0 && t.cell_valid)
+      for (int k = 0; k < t.tm_eff; ++k)
+        m |= kp.cmask[t.cell_index(kp, k)];
     atomicOr(&m_sh, m);
     __syncthreads();
     if (t.tid == 0)
@@ -182,6 +204,19 @@ namespace hps
 #pragma unroll
         for (int c = 0; c < NCA; ++c)
           interp_vg_hat<DIM>(pl[c], val[c], grad[c]);
+        // the tile's order parameters that the lane's cell drops
+        const unsigned cm = t.template cut_mask<true>(kp, k);
+        unsigned       on = 0u; // bit s: local order parameter s active here
+#pragma unroll
+        for (int s = 0; s < GA; ++s)
+          {
+            const bool a = (cm >> (gc[2 + s] - 2)) & 1u;
+            on |= (a ? 1u : 0u) << s;
+            val[2 + s] = a ? val[2 + s] : T(0);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              grad[2 + s][d] = a ? grad[2 + s][d] : T(0);
+          }
         __builtin_amdgcn_sched_barrier(0); // the physics waits for lin_next
 
         T *sl = &slots[t.corner][t.pn_lane * NCA];
@@ -208,6 +243,8 @@ namespace hps
                 gs[d] = c == 0 ? (M * fs.kj[d]) * grad[1][d] + sm * (lmu[d] * fs.gsj[d])
                                : (c == 1 ? fs.kmu[d] * grad[0][d] : fs.keta[d] * grad[c][d]);
               sl[c] = integrate_vg_hat<DIM>(vr[c] * jxw_hat, gs);
+              if (c >= 2 && !((on >> (c - 2)) & 1u))
+                sl[c] = T(0);
             }
         }
         __syncthreads();

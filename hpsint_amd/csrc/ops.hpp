@@ -28,6 +28,22 @@ namespace hps
                   hipStream_t);
     void (*probe32)(const KParams &, int n_tiles, const float *cache, float *dst, float *part,
                     int column, int ostride, int ocstride, int ooffset, hipStream_t);
+    // the same operations with the grain cut-off (per-cell masks kp.cmask)
+    void (*vmult64_cut)(const KParams &, int n_tiles, const double *src, double *dst,
+                        double *part, const double *cache, hipStream_t);
+    void (*vmult32_cut)(const KParams &, int n_tiles, const float *src, float *dst,
+                        float *part, const float *cache, hipStream_t);
+    void (*residual_cut)(const KParams &, int n_tiles, int wtd, const double *src,
+                         const double *o0, const double *o1, const double *o2,
+                         double *dst, double *part, hipStream_t);
+    void (*fill_cut)(const KParams &, int n_tiles, const double *lin, double *c64,
+                     float *c32, hipStream_t);
+    void (*probe_cut)(const KParams &, int n_tiles, const double *cache, double *dst,
+                      double *part, int column, int ostride, int ocstride, int ooffset,
+                      hipStream_t);
+    void (*probe32_cut)(const KParams &, int n_tiles, const float *cache, float *dst,
+                        float *part, int column, int ostride, int ocstride, int ooffset,
+                        hipStream_t);
   };
 
   template <int DIM, int TX, int TY, int G, bool TENS>
@@ -35,53 +51,73 @@ namespace hps
   {
     static constexpr int NT = TX * TY * (1 << DIM);
 
+    template <bool CUT>
     static void vmult64(const KParams &kp, int n_tiles, const double *src, double *dst,
                         double *part, const double *cache, hipStream_t s)
     {
-      q1_apply<DIM, TX, TY, G, TENS, OP_VMULT, double>
+      q1_apply<DIM, TX, TY, G, TENS, OP_VMULT, double, CUT>
         <<<n_tiles, NT, 0, s>>>(kp, src, nullptr, nullptr, nullptr, cache, dst, part);
     }
+    template <bool CUT>
     static void vmult32(const KParams &kp, int n_tiles, const float *src, float *dst,
                         float *part, const float *cache, hipStream_t s)
     {
-      q1_apply<DIM, TX, TY, G, TENS, OP_VMULT, float>
+      q1_apply<DIM, TX, TY, G, TENS, OP_VMULT, float, CUT>
         <<<n_tiles, NT, 0, s>>>(kp, src, nullptr, nullptr, nullptr, cache, dst, part);
     }
+    template <bool CUT>
     static void residual(const KParams &kp, int n_tiles, int wtd, const double *src,
                          const double *o0, const double *o1, const double *o2,
                          double *dst, double *part, hipStream_t s)
     {
       if (wtd == 0)
-        q1_apply<DIM, TX, TY, G, TENS, OP_RES0, double>
+        q1_apply<DIM, TX, TY, G, TENS, OP_RES0, double, CUT>
           <<<n_tiles, NT, 0, s>>>(kp, src, o0, o1, o2, nullptr, dst, part);
       else if (wtd == 1)
-        q1_apply<DIM, TX, TY, G, TENS, OP_RES1, double>
+        q1_apply<DIM, TX, TY, G, TENS, OP_RES1, double, CUT>
           <<<n_tiles, NT, 0, s>>>(kp, src, o0, o1, o2, nullptr, dst, part);
       else
-        q1_apply<DIM, TX, TY, G, TENS, OP_RES2, double>
+        q1_apply<DIM, TX, TY, G, TENS, OP_RES2, double, CUT>
           <<<n_tiles, NT, 0, s>>>(kp, src, o0, o1, o2, nullptr, dst, part);
     }
+    template <bool CUT>
     static void fill(const KParams &kp, int n_tiles, const double *lin, double *c64,
                      float *c32, hipStream_t s)
     {
-      q1_fill<DIM, TX, TY, G, TENS><<<n_tiles, NT, 0, s>>>(kp, lin, c64, c32);
+      q1_fill<DIM, TX, TY, G, TENS, CUT><<<n_tiles, NT, 0, s>>>(kp, lin, c64, c32);
     }
+    template <bool CUT>
     static void probe(const KParams &kp, int n_tiles, const double *cache, double *dst,
                       double *part, int column, int os, int ocs, int oo, hipStream_t s)
     {
-      q1_probe<DIM, TX, TY, G, TENS, double>
+      q1_probe<DIM, TX, TY, G, TENS, double, CUT>
         <<<n_tiles, NT, 0, s>>>(kp, cache, dst, part, column, os, ocs, oo);
     }
+    template <bool CUT>
     static void probe32(const KParams &kp, int n_tiles, const float *cache, float *dst,
                         float *part, int column, int os, int ocs, int oo, hipStream_t s)
     {
-      q1_probe<DIM, TX, TY, G, TENS, float>
+      q1_probe<DIM, TX, TY, G, TENS, float, CUT>
         <<<n_tiles, NT, 0, s>>>(kp, cache, dst, part, column, os, ocs, oo);
     }
     static OpTable table()
     {
-      return OpTable{NT,       TX,       TY,   CacheLayout<DIM, G, TENS>::n_fields,
-                     &vmult64, &vmult32, &residual, &fill, &probe, &probe32};
+      return OpTable{NT,
+                     TX,
+                     TY,
+                     CacheLayout<DIM, G, TENS>::n_fields,
+                     &vmult64<false>,
+                     &vmult32<false>,
+                     &residual<false>,
+                     &fill<false>,
+                     &probe<false>,
+                     &probe32<false>,
+                     &vmult64<true>,
+                     &vmult32<true>,
+                     &residual<true>,
+                     &fill<true>,
+                     &probe<true>,
+                     &probe32<true>};
     }
   };
 
@@ -217,12 +253,14 @@ namespace hps
   template <typename T>
   void launch_fixup(const FixParams &fp, const T *part, T *dst, hipStream_t s);
 
-  // grain cut-off (3D, scalar mobility, TX3 x TY3 tiles): per-tile masks of
-  // the active order parameters, and the tangent vmult of `count` tiles
-  // (tiles[]) that all have ga active order parameters
+  // grain cut-off: per-cell masks of the active order parameters (Q1, 2D
+  // or 3D); the union of the cell masks of every tile (3D, TX3 x TY3
+  // tiles); the fast tangent vmult of `count` tiles (tiles[]) that all have
+  // ga active order parameters (3D scalar mobility)
   constexpr int CUT_MAX_GA = 8;
-  void tile_masks_d3(const KParams &kp, int n_tiles, int n_grains, double tol, const double *u,
-                     unsigned *masks, hipStream_t s);
+  void cell_masks(const KParams &kp, int dim, int n_grains, double tol, const double *u,
+                  unsigned *masks, hipStream_t s);
+  void tile_masks_d3(const KParams &kp, int n_tiles, unsigned *masks, hipStream_t s);
   bool vmult_cut_d3(int ga, const KParams &kp, int n_grains, const int *tiles, int count,
                     const unsigned *masks, const double *src, const double *cache, double *dst,
                     double *part, hipStream_t s);

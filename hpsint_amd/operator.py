@@ -169,6 +169,7 @@ class SinteringOperatorGeneric:
         self.time_data = None
         self._order = 0
         self._old = []
+        self.cut_off_tolerance = 0.0
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -262,11 +263,28 @@ class SinteringOperatorGeneric:
             self._h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(arr.size)))
 
     def set_grain_cut_off(self, tol):
-        """grain cut-off of the tangent (set_component_mask,
-        S/operator_sintering_data.h:150-298): per tile, the order parameters
-        with |eta| <= tol at all its nodes are dropped from the tangent;
-        takes effect at the next fill_quadrature_point_values; tol < 0: off"""
+        """grain cut-off tolerance (GrainCutOffTolerance, S/parameters.h:314):
+        tol > 0 enables the cut-off, tol <= 0 disables it; the component
+        table comes from set_component_mask"""
         check(lib().hps_set_grain_cut_off(self._h, float(tol)))
+        self.cut_off_tolerance = float(tol) if tol > 0 else 0.0
+
+    def set_component_mask(self, u):
+        """SinteringOperatorData::set_component_mask (S/operator_sintering_
+        data.h:150-298): per cell, the order parameters with max |eta| over
+        the cell's nodes above the tolerance stay in the cell's residual,
+        tangent and preconditioner; the others are dropped.  The
+        linearisation must be set again afterwards."""
+        check(lib().hps_set_component_mask(self._h, _ptr(u)))
+
+    def component_mask(self):
+        """the component table: one int32 bit mask per cell (cells x fastest)"""
+        n = 1
+        for c in self.mesh.n_cells:
+            n *= c
+        out = torch.empty(n, dtype=torch.int32, device=self.device)
+        check(lib().hps_get_component_mask(self._h, _ptr(out)))
+        return out
 
     def vmult(self, dst, src):
         if src.dtype == torch.float32:

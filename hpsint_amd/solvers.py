@@ -124,6 +124,10 @@ class TimeStepper:
                 torch.cuda.synchronize()
                 tp = time.perf_counter() - tp0
             self.last_guess = x.clone() if self.keep_guess else None
+            if self.op.cut_off_tolerance > 0.0:
+                # the component table of the step's initial guess
+                # (S/driver.h:2212-2218)
+                self.op.set_component_mask(x)
             try:
                 st = self.solver.solve(x, old, r0_ref)
             except (_ffi.NewtonDidNotConverge, _ffi.LinearSolverNoConvergence):
@@ -144,3 +148,62 @@ class TimeStepper:
                          st.residual_evaluations, sec, tp)
         self.records.append(rec)
         return rec
+
+
+class ExtrapolationPredictor:
+    """The reference's predictors of the Newton initial guess
+    (Parameters::TimeIntegrationData::predictor, S/driver.h:2002-2050):
+
+    - "Euler": u + (-dt) R_0(u), one explicit Euler step with the residual
+      without time terms (evaluate_nonlinear_residual<0>) and sadd;
+    - "Midpoint": the explicit midpoint rule, two such residuals;
+    - "Linear": u_n + (dt_n / dt_{n-1}) (u_n - u_{n-1}), the solution
+      history's extrapolation (SolutionHistory::extrapolate).
+
+    All vector work runs in the library (residual, hps_sadd, hps_axpy).
+    Called by TimeStepper as predictor(x, states, dt) with x = u_n on entry
+    and states = [u_n, u_{n-1}, ...]."""
+
+    def __init__(self, op, kind="Euler"):
+        if kind not in ("Euler", "Midpoint", "Linear"):
+            raise ValueError("predictor must be Euler, Midpoint or Linear")
+        self.op = op
+        self.kind = kind
+        self._r = None
+        self._m = None
+        self.dt_prev = None
+
+    def _buf(self, name, like):
+        b = getattr(self, name)
+        if b is None or b.numel() != like.numel():
+            b = torch.empty_like(like)
+            setattr(self, name, b)
+        return b
+
+    def __call__(self, x, states, dt):
+        h, n = self.op.handle, x.numel()
+        p = ctypes.c_void_p
+        if len(states) < 2:
+            # the first step (t = 0) is not predicted
+            self.dt_prev = dt
+            return x
+        if self.kind == "Linear":
+            fac = dt / (self.dt_prev or dt)
+            self.dt_prev = dt
+            # x = u_n + fac (u_n - u_{n-1})
+            check(lib().hps_sadd(h, n, 1.0 + fac, -fac, p(states[1].data_ptr()),
+                                 p(x.data_ptr())))
+            return x
+        r = self._buf("_r", x)
+        # evaluate_nonlinear_residual<0> has no time terms: the weights do not matter
+        if self.kind == "Euler":
+            self.op.evaluate_nonlinear_residual(r, x, 0)
+            check(lib().hps_sadd(h, n, -dt, 1.0, p(x.data_ptr()), p(r.data_ptr())))
+        else:
+            m = self._buf("_m", x)
+            self.op.evaluate_nonlinear_residual(m, x, 0)
+            check(lib().hps_sadd(h, n, -dt / 2.0, 1.0, p(x.data_ptr()), p(m.data_ptr())))
+            self.op.evaluate_nonlinear_residual(r, m, 0)
+            check(lib().hps_sadd(h, n, -dt, 1.0, p(x.data_ptr()), p(r.data_ptr())))
+        check(lib().hps_copy(h, n, p(x.data_ptr()), p(r.data_ptr())))
+        return x

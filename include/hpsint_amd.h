@@ -120,16 +120,27 @@ int hps_residual(hps_operator op, int wtd, double *dst, const double *src,
  * preconditioners use diagonal 1 and identity node blocks for them. */
 int hps_set_constrained_nodes(hps_operator op, const int64_t *nodes, int64_t n);
 
-/* Grain cut-off of the tangent vmult: SinteringOperatorData::
- * set_component_mask (S/operator_sintering_data.h:150-298) with
- * do_vmult_range_no_template (S/operator_sintering_generic.h:456-537).
- * tol >= 0: at each hps_set_linearization the order parameters with
- * |eta| <= tol at every node of a tile (the cells one workgroup processes:
- * 8 x 4 cells x the march length) are dropped from that tile's part of the
- * tangent; tol < 0: off (the default).  Q1, 3D, scalar mobility; a tile with
- * more than 8 active order parameters makes that linearisation use the full
- * vmult.  The residual is never cut. */
+/* Grain cut-off (Parameters::grain_cut_off_tolerance, S/parameters.h:314):
+ * tol > 0 enables it (the reference enables it for a nonzero tolerance),
+ * tol <= 0 disables it and drops the component table.  Q1 operators (2D and
+ * 3D, scalar and tensorial mobility); Q2 returns HPS_ERR_UNSUPPORTED. */
 int hps_set_grain_cut_off(hps_operator op, double tol);
+/* SinteringOperatorData::set_component_mask (S/operator_sintering_data.h:
+ * 150-298), called by the driver at the start of every Newton solve
+ * (S/driver.h:2212-2218): the component table of the state u, one cell per
+ * batch -- bit g of a cell's mask is set when max |eta_g| over the cell's
+ * nodes exceeds the tolerance.  From then on the residual
+ * (do_evaluate_nonlinear_residual_nt, S/operator_sintering_generic.h:
+ * 1008-1081), the linearisation cache (zeros for the dropped order
+ * parameters, :360-395 of the data file), the tangent vmult
+ * (do_vmult_range_no_template, :456-537) and the preconditioner assembly
+ * evaluate every cell with c, mu and its active order parameters only; a
+ * component with no entry left in a node block gets a unit diagonal.  The
+ * linearisation must be set again after this call. */
+int hps_set_component_mask(hps_operator op, const double *u);
+/* copy of the component table: one uint32 per cell (cells x fastest) into
+ * the DEVICE array masks */
+int hps_get_component_mask(hps_operator op, uint32_t *masks);
 
 /* OperatorBase::vmult -> SinteringOperatorGeneric::vmult_internal
  * (S/operator_base.h:294-353, S/operator_sintering_generic.h:579-606):

@@ -697,6 +697,7 @@ typedef struct
   double             *dst;
   const double       *cache_in;
   double             *cache_out;
+  const uint32_t     *masks; /* grain cut-off: active order parameters per cell */
 } loop_args;
 
 static void do_cell(const loop_args *a, int cx, int cy, int cz)
@@ -806,6 +807,128 @@ static void do_cell(const loop_args *a, int cx, int cy, int cz)
       a->dst[idx[k] * nc + c] += r_loc[k * nc + c];
 }
 
+/* One cell of the grain cut-off (do_vmult_range_no_template and
+ * do_evaluate_nonlinear_residual_nt, S/operator_sintering_generic.h:456-537,
+ * 1008-1081): the cell is evaluated as a system of c, mu and its active order
+ * parameters only (bit g of masks[cell]); the linearisation values and
+ * gradients of the others are left out as the compacted cache of
+ * SinteringOperatorData::fill_quadrature_point_values leaves them out
+ * (S/operator_sintering_data.h:360-395), and their rows get nothing from
+ * the cell. */
+static void do_cell_cut(const loop_args *a, int cx, int cy, int cz)
+{
+  const orc_params *p   = a->p;
+  const fe_tables  *t   = a->t;
+  const int         nc  = p->n_comp;
+  const int         dim = p->dim;
+  int64_t           idx[MAXNL];
+  cell_dofs(t, cx, cy, cz, idx);
+  const int64_t  cell = cell_index(t, cx, cy, cz);
+  const uint32_t m    = a->masks[cell];
+  int            L[ORC_MAX_COMP], nl = 0;
+  L[nl++] = 0;
+  L[nl++] = 1;
+  for (int g = 0; g < nc - 2; ++g)
+    if ((m >> g) & 1u)
+      L[nl++] = 2 + g;
+  orc_params pr = *p;
+  pr.n_comp     = nl;
+  const int fsz = nc * (1 + dim); /* fields per q of the full cache */
+
+  double lv[ORC_MAX_COMP], lg[ORC_MAX_COMP * 3];
+  double value[ORC_MAX_COMP], grad[ORC_MAX_COMP * 3];
+  double vr[ORC_MAX_COMP], gr[ORC_MAX_COMP * 3];
+  /* the linearisation point at q, reduced to the cell's components */
+#define ORC_REDUCED_LIN(q)                                              \
+  do                                                                    \
+    {                                                                   \
+      const double *lin_ = a->cache_in + (cell * t->nq + (q)) * fsz;    \
+      for (int i_ = 0; i_ < nl; ++i_)                                   \
+        {                                                               \
+          lv[i_] = lin_[L[i_]];                                         \
+          for (int d_ = 0; d_ < dim; ++d_)                              \
+            lg[i_ * dim + d_] = lin_[nc + L[i_] * dim + d_];            \
+        }                                                               \
+    }                                                                   \
+  while (0)
+
+  if (a->op == OP_BLOCKS)
+    {
+      for (int k = 0; k < t->nl; ++k)
+        for (int jj = 0; jj < nl; ++jj)
+          {
+            double col[ORC_MAX_COMP];
+            for (int i = 0; i < nl; ++i)
+              col[i] = 0.0;
+            for (int q = 0; q < t->nq; ++q)
+              {
+                ORC_REDUCED_LIN(q);
+                for (int i = 0; i < nl; ++i)
+                  {
+                    value[i] = 0.0;
+                    for (int d = 0; d < dim; ++d)
+                      grad[i * dim + d] = 0.0;
+                  }
+                value[jj] = t->S[q][k];
+                for (int d = 0; d < dim; ++d)
+                  grad[jj * dim + d] = t->G[q][k][d];
+                orc_tangent_qp(&pr, lv, lg, value, grad, vr, gr);
+                for (int i = 0; i < nl; ++i)
+                  {
+                    double s = t->S[q][k] * vr[i];
+                    for (int d = 0; d < dim; ++d)
+                      s += t->G[q][k][d] * gr[i * dim + d];
+                    col[i] += t->JxW[q] * s;
+                  }
+              }
+            for (int i = 0; i < nl; ++i)
+              a->dst[(idx[k] * nc + L[i]) * nc + L[jj]] += col[i];
+          }
+      return;
+    }
+
+  double u_loc[MAXNL * ORC_MAX_COMP], h_loc[MAXNL * ORC_MAX_COMP];
+  double r_loc[MAXNL * ORC_MAX_COMP];
+  for (int k = 0; k < t->nl; ++k)
+    for (int i = 0; i < nl; ++i)
+      u_loc[k * nl + i] = a->src[idx[k] * nc + L[i]];
+  if (a->op == OP_RESIDUAL && a->wtd == 2)
+    for (int k = 0; k < t->nl; ++k)
+      for (int i = 0; i < nl; ++i)
+        {
+          double v = 0.0;
+          for (int o = 0; o < p->order; ++o)
+            {
+              const double w = a->old[o][idx[k] * nc + L[i]] * p->w_hist[o];
+              v              = o == 0 ? w : v + w;
+            }
+          h_loc[k * nl + i] = v;
+        }
+  for (int k = 0; k < t->nl * nl; ++k)
+    r_loc[k] = 0.0;
+  for (int q = 0; q < t->nq; ++q)
+    {
+      interpolate(t, nl, q, u_loc, value, grad);
+      if (a->op == OP_RESIDUAL)
+        {
+          double hist[ORC_MAX_COMP];
+          if (a->wtd == 2)
+            interpolate(t, nl, q, h_loc, hist, NULL);
+          orc_residual_qp(&pr, a->wtd, value, grad, hist, vr, gr);
+        }
+      else
+        {
+          ORC_REDUCED_LIN(q);
+          orc_tangent_qp(&pr, lv, lg, value, grad, vr, gr);
+        }
+      integrate_add(t, nl, q, vr, gr, r_loc);
+    }
+#undef ORC_REDUCED_LIN
+  for (int k = 0; k < t->nl; ++k)
+    for (int i = 0; i < nl; ++i)
+      a->dst[idx[k] * nc + L[i]] += r_loc[k * nl + i];
+}
+
 /* cell loop in 2^dim parity colours: cells of one colour share no node, so
  * the scatter-add is race-free and the result independent of the number of
  * threads */
@@ -826,7 +949,11 @@ static void cell_loop(const loop_args *a)
           const int ix = (int)(i % mx);
           const int iy = (int)((i / mx) % my);
           const int iz = (int)(i / ((int64_t)mx * my));
-          do_cell(a, 2 * ix + ox, 2 * iy + oy, t->dim == 3 ? 2 * iz + oz : 0);
+          const int cz = t->dim == 3 ? 2 * iz + oz : 0;
+          if (a->masks)
+            do_cell_cut(a, 2 * ix + ox, 2 * iy + oy, cz);
+          else
+            do_cell(a, 2 * ix + ox, 2 * iy + oy, cz);
         }
     }
 }
@@ -844,7 +971,7 @@ void orc_residual(const orc_params *p, int wtd, const double *src,
   fe_tables t;
   make_tables(p, &t);
   zero(dst, orc_n_nodes(p) * p->n_comp);
-  loop_args a = {p, &t, OP_RESIDUAL, wtd, src, old, dst, NULL, NULL};
+  loop_args a = {p, &t, OP_RESIDUAL, wtd, src, old, dst, NULL, NULL, NULL};
   cell_loop(&a);
 }
 
@@ -852,7 +979,7 @@ void orc_fill_qp(const orc_params *p, const double *lin, double *cache)
 {
   fe_tables t;
   make_tables(p, &t);
-  loop_args a = {p, &t, OP_FILL, 0, lin, NULL, NULL, NULL, cache};
+  loop_args a = {p, &t, OP_FILL, 0, lin, NULL, NULL, NULL, cache, NULL};
   cell_loop(&a);
 }
 
@@ -862,7 +989,7 @@ void orc_vmult(const orc_params *p, const double *cache, const double *src,
   fe_tables t;
   make_tables(p, &t);
   zero(dst, orc_n_nodes(p) * p->n_comp);
-  loop_args a = {p, &t, OP_VMULT, 0, src, NULL, dst, cache, NULL};
+  loop_args a = {p, &t, OP_VMULT, 0, src, NULL, dst, cache, NULL, NULL};
   cell_loop(&a);
 }
 
@@ -871,7 +998,7 @@ void orc_node_blocks(const orc_params *p, const double *cache, double *blocks)
   fe_tables t;
   make_tables(p, &t);
   zero(blocks, orc_n_nodes(p) * p->n_comp * p->n_comp);
-  loop_args a = {p, &t, OP_BLOCKS, 0, NULL, NULL, blocks, cache, NULL};
+  loop_args a = {p, &t, OP_BLOCKS, 0, NULL, NULL, blocks, cache, NULL, NULL};
   cell_loop(&a);
 }
 
@@ -890,4 +1017,66 @@ void orc_dense_tangent(const orc_params *p, const double *cache, double *mat)
     }
   free(e);
   free(col);
+}
+
+/* ---- grain cut-off ---- */
+
+void orc_cell_masks(const orc_params *p, const double *u, double tol, uint32_t *masks)
+{
+  fe_tables t;
+  make_tables(p, &t);
+  const int     nc = p->n_comp;
+  const int64_t n  = orc_n_cells(p);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < n; ++c)
+    {
+      const int cx = (int)(c % t.nc[0]);
+      const int cy = (int)((c / t.nc[0]) % t.nc[1]);
+      const int cz = p->dim == 3 ? (int)(c / ((int64_t)t.nc[0] * t.nc[1])) : 0;
+      int64_t   idx[MAXNL];
+      cell_dofs(&t, cx, cy, cz, idx);
+      uint32_t m = 0u;
+      for (int g = 0; g < nc - 2; ++g)
+        {
+          double linf = 0.0;
+          for (int k = 0; k < t.nl; ++k)
+            {
+              const double v = fabs(u[idx[k] * nc + 2 + g]);
+              linf           = v > linf ? v : linf;
+            }
+          if (linf > tol)
+            m |= 1u << g;
+        }
+      masks[c] = m;
+    }
+}
+
+void orc_vmult_cut(const orc_params *p, const double *cache, const uint32_t *masks,
+                   const double *src, double *dst)
+{
+  fe_tables t;
+  make_tables(p, &t);
+  zero(dst, orc_n_nodes(p) * p->n_comp);
+  loop_args a = {p, &t, OP_VMULT, 0, src, NULL, dst, cache, NULL, masks};
+  cell_loop(&a);
+}
+
+void orc_residual_cut(const orc_params *p, int wtd, const uint32_t *masks, const double *src,
+                      const double *const *old, double *dst)
+{
+  fe_tables t;
+  make_tables(p, &t);
+  zero(dst, orc_n_nodes(p) * p->n_comp);
+  loop_args a = {p, &t, OP_RESIDUAL, wtd, src, old, dst, NULL, NULL, masks};
+  cell_loop(&a);
+}
+
+void orc_node_blocks_cut(const orc_params *p, const double *cache, const uint32_t *masks,
+                         double *blocks)
+{
+  fe_tables t;
+  make_tables(p, &t);
+  zero(blocks, orc_n_nodes(p) * p->n_comp * p->n_comp);
+  loop_args a = {p, &t, OP_BLOCKS, 0, NULL, NULL, blocks, cache, NULL, masks};
+  cell_loop(&a);
 }

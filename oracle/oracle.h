@@ -115,6 +115,23 @@ void orc_node_blocks(const orc_params *p, const double *cache,
 void orc_dense_tangent(const orc_params *p, const double *cache,
                        double *mat);
 
+/* ---- grain cut-off (S/operator_sintering_data.h:150-298,
+ * S/operator_sintering_generic.h:456-537, 1008-1081) ---- */
+
+/* SinteringOperatorData::set_component_mask with one cell per batch: bit g
+ * of masks[cell] (cells x fastest) is set when the l-infinity norm of
+ * eta_g over the cell's nodes exceeds tol. */
+void orc_cell_masks(const orc_params *p, const double *u, double tol, uint32_t *masks);
+/* tangent vmult, residual and node blocks with each cell evaluated as the
+ * system of c, mu and its active order parameters (cache from
+ * orc_fill_qp of the full state) */
+void orc_vmult_cut(const orc_params *p, const double *cache, const uint32_t *masks,
+                   const double *src, double *dst);
+void orc_residual_cut(const orc_params *p, int wtd, const uint32_t *masks, const double *src,
+                      const double *const *old, double *dst);
+void orc_node_blocks_cut(const orc_params *p, const double *cache, const uint32_t *masks,
+                         double *blocks);
+
 /* OpenMP threads used by the cell loops (1 when built without OpenMP) */
 int  orc_max_threads(void);
 void orc_set_threads(int n);

@@ -69,6 +69,11 @@ def lib():
         L.orc_vmult.argtypes = [P, dp, dp, dp]
         L.orc_node_blocks.argtypes = [P, dp, dp]
         L.orc_dense_tangent.argtypes = [P, dp, dp]
+        up = ctypes.POINTER(ctypes.c_uint32)
+        L.orc_cell_masks.argtypes = [P, dp, ctypes.c_double, up]
+        L.orc_vmult_cut.argtypes = [P, dp, up, dp, dp]
+        L.orc_residual_cut.argtypes = [P, ctypes.c_int, up, dp, ctypes.POINTER(dp), dp]
+        L.orc_node_blocks_cut.argtypes = [P, dp, up, dp]
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
@@ -220,7 +225,58 @@ def residual_constrained(p, wtd, src, old, nodes):
     return dst.ravel()
 
 
-def vmult_cut(p, lin, src, cell_masks):
+def _up(a):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def cell_masks(p, u, tol):
+    """SinteringOperatorData::set_component_mask (S/operator_sintering_data.h:
+    150-298) with one cell per batch: bit g when max |eta_g| over the cell's
+    nodes > tol (uint32 per cell, cells x fastest)"""
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    m = np.empty(n_cells(p), dtype=np.uint32)
+    lib().orc_cell_masks(ctypes.byref(p), _dp(u), float(tol), _up(m))
+    return m
+
+
+def vmult_cut(p, cache, src, masks):
+    """tangent vmult with the grain cut-off (each cell is the system of c, mu
+    and its active order parameters, do_vmult_range_no_template,
+    S/operator_sintering_generic.h:456-537); cache from fill_qp of the full
+    state"""
+    src = np.ascontiguousarray(src, dtype=np.float64).ravel()
+    masks = np.ascontiguousarray(masks, dtype=np.uint32)
+    dst = np.empty_like(src)
+    lib().orc_vmult_cut(ctypes.byref(p), _dp(cache), _up(masks), _dp(src), _dp(dst))
+    return dst
+
+
+def residual_cut(p, wtd, src, old, masks):
+    """nonlinear residual with the grain cut-off
+    (do_evaluate_nonlinear_residual_nt, S/operator_sintering_generic.h:
+    1008-1081)"""
+    src = np.ascontiguousarray(src, dtype=np.float64).ravel()
+    masks = np.ascontiguousarray(masks, dtype=np.uint32)
+    dst = np.empty_like(src)
+    olds = [np.ascontiguousarray(o, dtype=np.float64).ravel() for o in old]
+    arr = (ctypes.POINTER(ctypes.c_double) * max(1, len(olds)))()
+    for i, o in enumerate(olds):
+        arr[i] = _dp(o)
+    lib().orc_residual_cut(ctypes.byref(p), wtd, _up(masks), _dp(src), arr, _dp(dst))
+    return dst
+
+
+def node_blocks_cut(p, cache, masks):
+    """node blocks of the cut tangent"""
+    masks = np.ascontiguousarray(masks, dtype=np.uint32)
+    nn = n_nodes(p)
+    out = np.empty(nn * p.n_comp * p.n_comp)
+    lib().orc_node_blocks_cut(ctypes.byref(p), _dp(cache), _up(masks), _dp(out))
+    return out.reshape(nn, p.n_comp, p.n_comp)
+
+
+def vmult_cut_py(p, lin, src, cell_masks):
     """tangent vmult with a grain cut-off, do_vmult_range_no_template
     (S/operator_sintering_generic.h:456-537): each cell is evaluated with c,
     mu and only the order parameters whose bit is set in cell_masks[cell]

@@ -459,67 +459,6 @@ def test_constrained_nodes_reject_bad_lists():
             op.set_constrained_nodes(bad)
 
 
-def tile_cell_masks(u, cells, G, tol, tile=(8, 4, 16)):
-    """the cut-off masks of the tiles (8 x 4 cells x 16 layers), given per
-    cell: bit g when |eta_g| > tol at some node of the cell's tile"""
-    nc = G + 2
-    nnx, nny, nnz = cells[0] + 1, cells[1] + 1, cells[2] + 1
-    eta = np.abs(u.reshape(nnz, nny, nnx, nc)[..., 2:])
-    masks = np.zeros(cells[0] * cells[1] * cells[2], dtype=np.int64)
-    for z0 in range(0, cells[2], tile[2]):
-        for y0 in range(0, cells[1], tile[1]):
-            for x0 in range(0, cells[0], tile[0]):
-                x1 = min(x0 + tile[0], cells[0])
-                y1 = min(y0 + tile[1], cells[1])
-                z1 = min(z0 + tile[2], cells[2])
-                box = eta[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1].reshape(-1, G).max(axis=0)
-                m = sum(1 << g for g in range(G) if box[g] > tol)
-                for z in range(z0, z1):
-                    for y in range(y0, y1):
-                        masks[x0 + cells[0] * (y + cells[1] * z):
-                              x1 + cells[0] * (y + cells[1] * z)] = m
-    return masks
-
-
-@pytest.mark.parametrize("G,tol", [(4, 0.0), (4, 1e-3), (3, 0.5), (4, 10.0)])
-def test_grain_cut_off_vmult_matches_oracle(G, tol):
-    """tangent with the grain cut-off against the oracle's reduced cell
-    systems (S/operator_sintering_generic.h:456-537), FP64 at 1e-12 and FP32
-    per field at 1e-5; order parameters vanish (exactly) outside slabs, so
-    tiles hold 0..G active order parameters"""
-    cells = (24, 12, 40)
-    op, p = make_pair(3, cells, G, w0=100.0)
-    nn = po.n_nodes(p)
-    rng = np.random.default_rng(5)
-    u = rng.uniform(0.0, 1.0, size=(nn, G + 2))
-    u[:, 1] = rng.uniform(-1, 1, nn)
-    x = np.arange(nn) % (cells[0] + 1)
-    z = np.arange(nn) // ((cells[0] + 1) * (cells[1] + 1))
-    for g in range(G):
-        inside = (np.abs(x - 6 * g - 3) <= 5) & (np.abs(z - 10 * g - 5) <= 9)
-        u[:, 2 + g] *= np.where(inside, 1.0, 0.0) + np.where(inside, 0.0, 1e-4)
-    src = rng.uniform(-1, 1, size=nn * (G + 2))
-    masks = tile_cell_masks(u.ravel(), cells, G, tol)
-    ref = po.vmult_cut(p, u.ravel(), src, masks)
-    op.set_grain_cut_off(tol)
-    op.fill_quadrature_point_values(dev(u.ravel()))
-    dst = op.initialize_dof_vector()
-    op.vmult(dst, dev(src))
-    assert rel_l2(host(dst), ref) <= TOL64, rel_l2(host(dst), ref)
-    d32 = op.initialize_dof_vector(torch.float32)
-    op.vmult(d32, dev(src, torch.float32))
-    g32 = host(d32).reshape(-1, G + 2)
-    r2 = ref.reshape(-1, G + 2)
-    errs = [rel_l2(g32[:, c], r2[:, c]) for c in range(G + 2) if np.abs(r2[:, c]).max() > 0]
-    assert max(errs) <= 1e-5, errs
-    # off again: the full tangent
-    op.set_grain_cut_off(-1.0)
-    op.fill_quadrature_point_values(dev(u.ravel()))
-    op.vmult(dst, dev(src))
-    full = po.vmult(p, po.fill_qp(p, u.ravel()), src)
-    assert rel_l2(host(dst), full) <= TOL64
-
-
 def test_every_grain_count_is_built():
     """hps_create accepts every grain count 1..12 for Q1 and Q2, 2D and 3D,
     scalar and tensorial mobility (EXPAND_OPERATIONS instantiates all of

@@ -257,7 +257,7 @@ def test_constrained_vmult_is_the_matrix_with_identity_rows(dim, degree, cells):
 
 
 def test_cut_off_vmult_is_the_reduced_system():
-    """vmult_cut (S/operator_sintering_generic.h:456-537): with every grain
+    """vmult_cut_py (S/operator_sintering_generic.h:456-537): with every grain
     active it is the full tangent; with grain g inactive in every cell its
     rows are zero and the other rows are the tangent of the system without
     g (one fewer order parameter)"""
@@ -268,12 +268,12 @@ def test_cut_off_vmult_is_the_reduced_system():
     u[:, 1] = rng.uniform(-1, 1, nn)
     src = rng.uniform(-1, 1, size=(nn, 6))
     full = po.vmult(p, po.fill_qp(p, u.ravel()), src.ravel())
-    cut = po.vmult_cut(p, u.ravel(), src.ravel(), np.full(12, 0b1111))
+    cut = po.vmult_cut_py(p, u.ravel(), src.ravel(), np.full(12, 0b1111))
     np.testing.assert_allclose(cut, full, rtol=0, atol=1e-13 * np.abs(full).max())
     g = 2
     keep = [c for c in range(6) if c != 2 + g]
     pr = po.make_params(3, 1, (3, 2, 2), (0.5, 0.75, 1.0), 5, w0=100.0)
     red = po.vmult(pr, po.fill_qp(pr, u[:, keep].ravel()), src[:, keep].ravel()).reshape(nn, 5)
-    cut = po.vmult_cut(p, u.ravel(), src.ravel(), np.full(12, 0b1111 & ~(1 << g))).reshape(nn, 6)
+    cut = po.vmult_cut_py(p, u.ravel(), src.ravel(), np.full(12, 0b1111 & ~(1 << g))).reshape(nn, 6)
     assert not cut[:, 2 + g].any()
     np.testing.assert_allclose(cut[:, keep], red, rtol=0, atol=1e-13 * np.abs(red).max())
