@@ -133,6 +133,7 @@ _SIGNATURES = {
     "hps_copy": [vp, i64, vp, vp],
     "hps_fill": [vp, i64, dbl, vp],
     "hps_multi_dot": [vp, i64, cint, vp, vp, pd],
+    "hps_multi_dot_noalloc": [vp, i64, cint, vp, vp, vp],
     "hps_multi_axpy_norm": [vp, i64, cint, vp, pd, vp, pd],
     "hps_convert_f64_to_f32": [vp, i64, vp, vp],
     "hps_norm2_f32": [vp, i64, vp, pd],

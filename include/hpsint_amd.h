@@ -189,6 +189,16 @@ int hps_fill(hps_operator op, int64_t n, double value, double *x);
  * basis V stored as k contiguous vectors of n entries (V + j*n) */
 int hps_multi_dot(hps_operator op, int64_t n, int k, const double *V,
                   const double *w, double *h);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hps_multi_dot without an operator handle.  h[j] =
+ * <V_j, w> for j < k, k <= 32, written to DEVICE memory h (k doubles) on
+ * `stream` (a hipStream_t; NULL = default stream) without waiting for it.
+ * The partial sums live in a fixed 512 KiB device array of the library (one
+ * per GPU, held until the process ends); calls that overlap on one GPU race
+ * on it.  Allocates nothing; k > 32 returns HPS_ERR_INVALID_ARGUMENT. */
+int hps_multi_dot_noalloc(void *stream, int64_t n, int k, const double *V, const double *w,
+                          double *h);
 /* w -= sum_j h[j] V_j, then returns ||w|| (fused CGS update + norm) */
 int hps_multi_axpy_norm(hps_operator op, int64_t n, int k, const double *V,
                         const double *h, double *w, double *norm);
