@@ -147,6 +147,8 @@ _SIGNATURES = {
     "hps_to_blocks": [vp, ctypes.POINTER(vp), vp],
     "hps_last_kernel_ms": [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "hps_enable_timing": [vp, cint],
+    "hps_kernel_ms_mean": [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                           ctypes.POINTER(cint)],
     "hps_newton_create": [vp, ctypes.POINTER(HpsNewtonParams), ctypes.POINTER(vp)],
     "hps_newton_destroy": [vp],
     "hps_newton_clear": [vp],

@@ -82,7 +82,10 @@ struct hps_operator_s
   double              *diag, *inv_diag64, *blocks, *inv_blocks64;
   float               *inv_diag32, *inv_blocks32;
   int                 *d_singular;
-  hipEvent_t           ev[3];
+  // kernel timing: a ring of `tcap` event triples (before the main kernel,
+  // between it and the fix-up, after the fix-up); tcount calls recorded
+  std::vector<hipEvent_t> *tev;
+  int                  tcap, tcount;
   bool                 timing;
   const double        *old[3];
   double               cut_tol;          // grain cut-off tolerance (<= 0: off)
@@ -113,6 +116,17 @@ static hps::Q2Params q2_params(hps_operator op)
   for (int i = 0; i < 3; ++i)
     q.w_hist[i] = op->kp.w_hist[i];
   return q;
+}
+
+// event i (0: before the main kernel, 1: before the fix-up, 2: after it) of
+// the current call's triple; i == 2 completes the call
+static hipEvent_t timing_event(hps_operator op, int i)
+{
+  const int slot = op->tcount % op->tcap;
+  hipEvent_t ev  = (*op->tev)[3 * slot + i];
+  if (i == 2)
+    ++op->tcount;
+  return ev;
 }
 
 int hps_internal_fail(int code, const char *msg)
@@ -282,9 +296,7 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
     return cleanup_fail(e);
   if ((e = hipMalloc(&op->d_singular, sizeof(int))) != hipSuccess)
     return cleanup_fail(e);
-  for (int i = 0; i < 3; ++i)
-    if ((e = hipEventCreate(&op->ev[i])) != hipSuccess)
-      return cleanup_fail(e);
+  op->tev = new std::vector<hipEvent_t>();
   *out = op;
   return HPS_OK;
 }
@@ -316,9 +328,12 @@ extern "C" int hps_destroy(hps_operator op)
   (void)hipFree(op->cut_masks);
   (void)hipFree(op->cut_tiles);
   (void)hipFree(op->cell_masks);
-  for (int i = 0; i < 3; ++i)
-    if (op->ev[i])
-      (void)hipEventDestroy(op->ev[i]);
+  if (op->tev)
+    {
+      for (hipEvent_t ev : *op->tev)
+        (void)hipEventDestroy(ev);
+      delete op->tev;
+    }
   delete op;
   return HPS_OK;
 }
@@ -526,7 +541,7 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
         }
     }
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[0], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 0), op->stream));
   // with order 0 there is no history: evaluate without the history term
   const int w = (wtd == 2 && op->kp.order == 0) ? 1 : wtd;
   if (op->q2)
@@ -536,11 +551,11 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
       op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[1], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
   if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0))
     return r;
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));
   // FEEvaluation::distribute_local_to_global leaves constrained entries
   // untouched: their residual rows are zero
   if (op->n_constrained)
@@ -600,7 +615,7 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
       HPS_LAUNCHED();
     }
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[0], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 0), op->stream));
   if (op->q2)
     op->q2->vmult64(q2_params(op), src, dst, op->cache64, op->stream);
   else if (op->cut_ready)
@@ -617,11 +632,11 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
                                                                op->stream);
   HPS_LAUNCHED();
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[1], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
   if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0))
     return r;
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));
   if (op->n_constrained)
     {
       hps::constraint_restore64(op->stream, op->n_constrained, op->nc, op->constrained,
@@ -647,7 +662,7 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
       HPS_LAUNCHED();
     }
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[0], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 0), op->stream));
   if (op->q2)
     op->q2->vmult32(q2_params(op), src, dst, op->cache32, op->stream);
   else if (op->cut_ready)
@@ -664,11 +679,11 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
       op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[1], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
   if (int r = finish_fixup<float>(op, dst, op->nc, 1, 0))
     return r;
   if (op->timing)
-    HPS_HIP(hipEventRecord(op->ev[2], op->stream));
+    HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));
   if (op->n_constrained)
     {
       hps::constraint_restore32(op->stream, op->n_constrained, op->nc, op->constrained,
@@ -1023,7 +1038,20 @@ extern "C" int hps_enable_timing(hps_operator op, int enable)
 {
   if (int r = check_op(op))
     return r;
+  if (enable < 0 || enable > 4096)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "timing ring: 0..4096 calls");
+  HPS_HIP(hipStreamSynchronize(op->stream));
   op->timing = enable != 0;
+  op->tcount = 0;
+  if (!op->timing)
+    return HPS_OK;
+  while ((int)op->tev->size() < 3 * enable)
+    {
+      hipEvent_t ev;
+      HPS_HIP(hipEventCreate(&ev));
+      op->tev->push_back(ev);
+    }
+  op->tcap = enable;
   return HPS_OK;
 }
 
@@ -1031,16 +1059,47 @@ extern "C" int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fix
 {
   if (int r = check_op(op))
     return r;
-  if (!op->timing)
-    return fail(HPS_ERR_NOT_INITIALIZED, "timing not enabled");
-  HPS_HIP(hipEventSynchronize(op->ev[2]));
+  if (!op->timing || op->tcount == 0)
+    return fail(HPS_ERR_NOT_INITIALIZED, "timing not enabled or nothing timed");
+  const int   slot = (op->tcount - 1) % op->tcap;
+  hipEvent_t *e    = op->tev->data() + 3 * slot;
+  HPS_HIP(hipEventSynchronize(e[2]));
   float a = 0.f, b = 0.f;
-  HPS_HIP(hipEventElapsedTime(&a, op->ev[0], op->ev[1]));
-  HPS_HIP(hipEventElapsedTime(&b, op->ev[1], op->ev[2]));
+  HPS_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+  HPS_HIP(hipEventElapsedTime(&b, e[1], e[2]));
   if (ms_main)
     *ms_main = a;
   if (ms_fixup)
     *ms_fixup = b;
+  return HPS_OK;
+}
+
+extern "C" int hps_kernel_ms_mean(hps_operator op, float *ms_main, float *ms_fixup, int *count)
+{
+  if (int r = check_op(op))
+    return r;
+  if (!op->timing || op->tcount == 0)
+    return fail(HPS_ERR_NOT_INITIALIZED, "timing not enabled or nothing timed");
+  const int n = op->tcount < op->tcap ? op->tcount : op->tcap;
+  HPS_HIP(hipStreamSynchronize(op->stream));
+  double sa = 0.0, sb = 0.0;
+  for (int k = 0; k < n; ++k)
+    {
+      const int   slot = (op->tcount - 1 - k) % op->tcap;
+      hipEvent_t *e    = op->tev->data() + 3 * slot;
+      float       a = 0.f, b = 0.f;
+      HPS_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+      HPS_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+      sa += a;
+      sb += b;
+    }
+  if (ms_main)
+    *ms_main = (float)(sa / n);
+  if (ms_fixup)
+    *ms_fixup = (float)(sb / n);
+  if (count)
+    *count = n;
+  op->tcount = 0;
   return HPS_OK;
 }
 

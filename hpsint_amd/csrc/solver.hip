@@ -339,6 +339,8 @@ extern "C" int hps_newton_create(hps_operator op, const hps_newton_params *prm,
     return fail(HPS_ERR_INVALID_ARGUMENT, "null argument");
   if (prm->l_restart < 1 || prm->l_restart > 31)
     return fail(HPS_ERR_INVALID_ARGUMENT, "GMRES restart length must be 1..31");
+  if (prm->inner_fp32 < 0 || prm->inner_fp32 > 2)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "inner_fp32 must be 0, 1 or 2");
   int64_t nn = 0, nd = 0;
   HPS_TRY(hps_sizes(op, &nn, &nd));
   hps_newton nt = new hps_newton_s();
@@ -354,11 +356,12 @@ extern "C" int hps_newton_create(hps_operator op, const hps_newton_params *prm,
   e = e ? e : hipMalloc(&nt->z64, b64);
   if (prm->inner_fp32)
     {
-      e = e ? e : hipMalloc(&nt->V32, b32 * (prm->l_restart + 1));
       e = e ? e : hipMalloc(&nt->x32, b32);
       e = e ? e : hipMalloc(&nt->y32, b32);
       e = e ? e : hipMalloc(&nt->z32, b32);
     }
+  if (prm->inner_fp32 == 1)
+    e = e ? e : hipMalloc(&nt->V32, b32 * (prm->l_restart + 1));
   else
     e = e ? e : hipMalloc(&nt->V, b64 * (prm->l_restart + 1));
   if (e != hipSuccess)
@@ -441,8 +444,10 @@ extern "C" int hps_newton_solve_ref(hps_newton nt, double *x, const double *cons
           ++s.precond_updates;
         }
       int lin = 0;
-      int rc  = p.inner_fp32 ? gmres_mixed(nt, nt->r, nt->inc, &lin)
-                             : gmres(nt, nt->r, nt->inc, &lin);
+      // inner_fp32 == 1: FP32 Krylov basis; 2: FP64 basis around the FP32
+      // tangent and preconditioner (apply_prec_op, apply_prec)
+      int rc  = p.inner_fp32 == 1 ? gmres_mixed(nt, nt->r, nt->inc, &lin)
+                                  : gmres(nt, nt->r, nt->inc, &lin);
       s.linear_iterations += lin;
       nt->history_linear_iterations_last = lin;
       if (rc != HPS_OK)

@@ -310,8 +310,18 @@ class SinteringOperatorGeneric:
         check(lib().hps_norm2(self._h, x.numel(), _ptr(x), ctypes.byref(r)))
         return r.value
 
-    def enable_timing(self, on=True):
-        check(lib().hps_enable_timing(self._h, int(on)))
+    def enable_timing(self, on=True, ring=1):
+        """HIP-event timing of the main kernel and the fix-up of every vmult and
+        residual, kept for the last `ring` calls without waiting"""
+        check(lib().hps_enable_timing(self._h, int(ring) if on else 0))
+
+    def kernel_ms_mean(self):
+        """(main, fix-up, count): mean kernel durations in ms over the calls
+        recorded since timing was enabled (at most `ring` of them)"""
+        a, b, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        check(lib().hps_kernel_ms_mean(self._h, ctypes.byref(a), ctypes.byref(b),
+                                       ctypes.byref(n)))
+        return a.value, b.value, n.value
 
     def last_kernel_ms(self):
         a, b = ctypes.c_float(), ctypes.c_float()

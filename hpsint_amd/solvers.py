@@ -40,6 +40,8 @@ class NewtonSolver:
         p.use_damping = int(use_damping)
         p.threshold_newton_iter = threshold_newton_iter
         p.threshold_linear_iter = threshold_linear_iter
+        # inner_fp32: False/0 FP64; True/1 FP32 tangent, preconditioner and
+        # Krylov basis; 2 FP32 tangent and preconditioner with an FP64 basis
         p.inner_fp32 = int(inner_fp32)
         self.params = p
         h = ctypes.c_void_p()

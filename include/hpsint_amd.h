@@ -231,11 +231,17 @@ int hps_axpy_multi_f32(hps_operator op, int64_t n, int k, const double *a, const
 int hps_from_blocks(hps_operator op, double *x, const double *const *blocks);
 int hps_to_blocks(hps_operator op, double *const *blocks, const double *x);
 
-/* duration in milliseconds of the last hps_vmult_* / hps_residual call as
- * measured by HIP events on the operator's stream around its kernels
- * (main kernel + shared-node fix-up); for measurement only */
-int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fixup);
+/* Kernel timing for measurement: with hps_enable_timing(op, n), n >= 1,
+ * every hps_vmult_* / hps_residual call records HIP events on the
+ * operator's stream around its main kernel and its shared-node fix-up, into
+ * a ring of the last n calls, without waiting (n = 0: off).
+ * hps_last_kernel_ms waits for the last call and returns its durations in
+ * milliseconds; hps_kernel_ms_mean waits for the stream and returns the
+ * mean durations over the recorded calls (at most n) and their number, then
+ * starts a new count. */
 int hps_enable_timing(hps_operator op, int enable);
+int hps_last_kernel_ms(hps_operator op, float *ms_main, float *ms_fixup);
+int hps_kernel_ms_mean(hps_operator op, float *ms_main, float *ms_fixup, int *count);
 
 /* ------------------------------------------------------------------
  * Newton-Krylov driver (host logic over the device operator).
@@ -262,7 +268,11 @@ typedef struct
   int    use_damping;
   int    threshold_newton_iter;
   int    threshold_linear_iter;
-  int    inner_fp32;             /* 1: FP32 vmult and preconditioner inside GMRES */
+  int    inner_fp32;             /* 0: FP64 throughout; 1: FP32 vmult, preconditioner
+                                    and Krylov basis storage (Gram-Schmidt
+                                    coefficients and norms accumulated in FP64);
+                                    2: FP32 vmult and preconditioner, FP64 basis,
+                                    orthogonalisation and norms */
 } hps_newton_params;
 
 typedef struct
