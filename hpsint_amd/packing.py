@@ -204,6 +204,50 @@ def initial_fields(p: Packing, n_cells, w=1.0, n_order_params=None, dtype=np.flo
     return out.ravel()
 
 
+def read_cloud(source, dim=3):
+    """read_particles (S/particle.h:38-86): a particle cloud in CSV form,
+    one header line, then one particle per line as "x,y,z,r"; the first
+    `dim` coordinates are the centre (a 2D run reads x, y), the fourth column
+    the radius; particles are numbered in file order.  `source` is a path or
+    an open text stream."""
+    if isinstance(source, str):
+        with open(source) as f:
+            return read_cloud(f, dim)
+    centers, radii = [], []
+    for k, line in enumerate(source):
+        if k == 0:
+            continue  # the header
+        line = line.strip()
+        if not line:
+            continue
+        cols = [c.strip() for c in line.split(",")]
+        centers.append([float(c) for c in cols[:dim]])
+        radii.append(float(cols[3]))
+    return np.array(centers, dtype=np.float64), np.array(radii, dtype=np.float64)
+
+
+def packing_from_cloud(source, dim=3, interface_width=2.0, boundary_factor=0.5,
+                       interface_buffer_ratio=1.0, radius_buffer_ratio=0.0):
+    """the packing of a particle cloud as the reference's InitialValuesCloud
+    and driver set it up (S/initial_values_cloud.h:30-260, S/driver.h:
+    505-525): domain = bounding box of the particles (centre +- radius)
+    enlarged by boundary_factor * the largest radius on every side; order
+    parameters from the interaction graph of InitialValuesCloud (particles
+    interact when their distance is at most r_i + r_j + interface_width (1 +
+    interface_buffer_ratio) + radius_buffer_ratio max(r_i, r_j)), coloured by
+    DSatur (the reference colours with SparsityTools::color_sparsity_pattern).
+    Defaults: GeometryData of S/parameters.h:57-75."""
+    centers, radii = read_cloud(source, dim)
+    rmax = float(radii.max())
+    lo = tuple(float(v) - boundary_factor * rmax for v in (centers - radii[:, None]).min(axis=0))
+    hi = tuple(float(v) + boundary_factor * rmax for v in (centers + radii[:, None]).max(axis=0))
+    p = Packing(centers, radii, lo, hi)
+    p.colors = greedy_coloring(interaction_graph(
+        p, w=interface_width, interface_buffer_ratio=interface_buffer_ratio,
+        radius_buffer_ratio=radius_buffer_ratio))
+    return p
+
+
 CONFIGS = {
     0: dict(name="2D two-particle neck growth", dim=2, n_cells=(160, 80),
             packing=two_particles_2d),

@@ -53,3 +53,46 @@ def test_config_order_parameter_counts():
     assert packing.two_particles_2d().n_order_params == 2
     # the predictor study builds one operator with 12 order parameters
     assert max(packing.packing_cfg4(seed=s).n_order_params for s in range(5)) <= 12
+
+
+def test_read_cloud_files(golden_dir):
+    """read_particles (S/particle.h:38-86) on clouds shipped with the
+    reference's examples (applications/sintering/sintering_cloud_examples):
+    one header line, x,y,z,r per particle in file order"""
+    import os
+    path = os.path.join(golden_dir, "clouds", "10particles.cloud")
+    rows = [l.strip().split(",") for l in open(path).read().splitlines()[1:] if l.strip()]
+    c3, r3 = packing.read_cloud(path, dim=3)
+    c2, r2 = packing.read_cloud(path, dim=2)
+    assert c3.shape == (10, 3) and c2.shape == (10, 2) and r3.shape == (10,)
+    assert np.array_equal(c3[:, :2], c2) and np.array_equal(r3, r2)
+    assert np.array_equal(c3, np.array([[float(v) for v in r[:3]] for r in rows]))
+    assert np.array_equal(r3, np.array([float(r[3]) for r in rows]))
+    assert tuple(c3[0]) == (7.816, 7.637, 0.0) and r3[0] == 7.5
+    c, r = packing.read_cloud(os.path.join(golden_dir, "clouds", "49particles.cloud"))
+    assert c.shape == (49, 3) and r[0] == 30.0706
+
+
+@pytest.mark.parametrize("name,dim", [("10particles", 2), ("3particles_overlap", 3),
+                                      ("49particles", 3)])
+def test_packing_from_cloud(golden_dir, name, dim):
+    """domain: bounding box of the particles enlarged by 0.5 r_max per side
+    (S/driver.h:505-525); order parameters: interacting particles (distance
+    <= r_i + r_j + 2 interface widths, S/initial_values_cloud.h:187-198)
+    get distinct colours, and the initial fields hold every particle"""
+    import os
+    pk = packing.packing_from_cloud(os.path.join(golden_dir, "clouds", name + ".cloud"), dim=dim)
+    rmax = pk.radii.max()
+    lo = (pk.centers - pk.radii[:, None]).min(axis=0) - 0.5 * rmax
+    hi = (pk.centers + pk.radii[:, None]).max(axis=0) + 0.5 * rmax
+    assert np.allclose(pk.lower, lo) and np.allclose(pk.upper, hi)
+    n = len(pk.radii)
+    for i in range(n):
+        for j in range(i + 1, n):
+            d = np.linalg.norm(pk.centers[i] - pk.centers[j])
+            if d <= pk.radii[i] + pk.radii[j] + 2.0 * 2.0:
+                assert pk.colors[i] != pk.colors[j], (i, j)
+    cells = (24,) * dim
+    u = packing.initial_fields(pk, cells).reshape(-1, pk.n_order_params + 2)
+    assert u[:, 2:].max() > 0.99
+    assert np.allclose(u[:, 0], u[:, 2:].sum(axis=1))
