@@ -1080,3 +1080,81 @@ void orc_node_blocks_cut(const orc_params *p, const double *cache, const uint32_
   loop_args a = {p, &t, OP_BLOCKS, 0, NULL, NULL, blocks, cache, NULL, masks};
   cell_loop(&a);
 }
+
+/* ---- incomplete LU without fill (ILU(0)), for the preconditioner study:
+ * the reference's default preconditioner of the Cahn-Hilliard block is
+ * Trilinos' ILU (BlockPreconditioner2, S/preconditioners.h:983-1282,
+ * PreconditionerData default "ILU", S/preconditioner_parameters.h:57-64).
+ * CSR with sorted column indices and a stored diagonal; factors in place
+ * (unit-lower L below the diagonal, U on and above it). */
+int orc_ilu0(int64_t n, const int64_t *rowptr, const int64_t *col, double *val)
+{
+  int64_t *diag = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+  int64_t *pos  = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+  if (!diag || !pos)
+    {
+      free(diag);
+      free(pos);
+      return -1;
+    }
+  for (int64_t i = 0; i < n; ++i)
+    {
+      diag[i] = -1;
+      pos[i]  = -1;
+    }
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (col[k] == i)
+        diag[i] = k;
+  int status = 0;
+  for (int64_t i = 0; i < n && status == 0; ++i)
+    {
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+        pos[col[k]] = k;
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1] && col[k] < i; ++k)
+        {
+          const int64_t j = col[k];
+          if (diag[j] < 0 || val[diag[j]] == 0.0)
+            {
+              status = -2;
+              break;
+            }
+          const double f = val[k] / val[diag[j]];
+          val[k]         = f;
+          for (int64_t m = diag[j] + 1; m < rowptr[j + 1]; ++m)
+            if (pos[col[m]] >= 0)
+              val[pos[col[m]]] -= f * val[m];
+        }
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+        pos[col[k]] = -1;
+      if (diag[i] < 0)
+        status = -2;
+    }
+  free(diag);
+  free(pos);
+  return status;
+}
+
+/* x = (LU)^-1 b with the factors of orc_ilu0 */
+void orc_ilu0_solve(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val,
+                    const double *b, double *x)
+{
+  for (int64_t i = 0; i < n; ++i)
+    {
+      double s = b[i];
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1] && col[k] < i; ++k)
+        s -= val[k] * x[col[k]];
+      x[i] = s;
+    }
+  for (int64_t i = n - 1; i >= 0; --i)
+    {
+      double  s = x[i];
+      int64_t d = -1;
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+        if (col[k] > i)
+          s -= val[k] * x[col[k]];
+        else if (col[k] == i)
+          d = k;
+      x[i] = s / val[d];
+    }
+}

@@ -132,6 +132,12 @@ void orc_residual_cut(const orc_params *p, int wtd, const uint32_t *masks, const
 void orc_node_blocks_cut(const orc_params *p, const double *cache, const uint32_t *masks,
                          double *blocks);
 
+/* ILU(0) of a CSR matrix (sorted columns, diagonal stored), in place;
+ * returns 0, or < 0 on a missing or zero pivot; and its solve */
+int  orc_ilu0(int64_t n, const int64_t *rowptr, const int64_t *col, double *val);
+void orc_ilu0_solve(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val,
+                    const double *b, double *x);
+
 /* OpenMP threads used by the cell loops (1 when built without OpenMP) */
 int  orc_max_threads(void);
 void orc_set_threads(int n);

@@ -74,6 +74,10 @@ def lib():
         L.orc_vmult_cut.argtypes = [P, dp, up, dp, dp]
         L.orc_residual_cut.argtypes = [P, ctypes.c_int, up, dp, ctypes.POINTER(dp), dp]
         L.orc_node_blocks_cut.argtypes = [P, dp, up, dp]
+        ip = ctypes.POINTER(ctypes.c_int64)
+        L.orc_ilu0.argtypes = [ctypes.c_int64, ip, ip, dp]
+        L.orc_ilu0.restype = ctypes.c_int
+        L.orc_ilu0_solve.argtypes = [ctypes.c_int64, ip, ip, dp, dp, dp]
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
@@ -347,3 +351,27 @@ def max_threads():
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+class ILU0:
+    """ILU(0) of a scipy CSR matrix (the factors and the solve in C)"""
+
+    def __init__(self, A):
+        A = A.tocsr().copy()
+        A.sort_indices()
+        self.n = A.shape[0]
+        self.rowptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(A.indices, dtype=np.int64)
+        self.val = np.ascontiguousarray(A.data, dtype=np.float64)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        self._r = self.rowptr.ctypes.data_as(ip)
+        self._c = self.col.ctypes.data_as(ip)
+        rc = lib().orc_ilu0(self.n, self._r, self._c, _dp(self.val))
+        if rc != 0:
+            raise ValueError("ILU(0): zero or missing pivot")
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(b)
+        lib().orc_ilu0_solve(self.n, self._r, self._c, _dp(self.val), _dp(b), _dp(x))
+        return x
