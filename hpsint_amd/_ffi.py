@@ -26,6 +26,7 @@ HPS_MOBILITY_SCALAR = 0
 HPS_MOBILITY_TENSORIAL = 1
 HPS_PRECON_POINT_JACOBI = 0
 HPS_PRECON_NODE_BLOCK_JACOBI = 1
+HPS_PRECON_BLOCK_CH_ILU = 2
 
 
 class HpsParams(ctypes.Structure):

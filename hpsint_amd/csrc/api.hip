@@ -12,6 +12,7 @@
 
 #include "../../include/hpsint_amd.h"
 #include "blas.hpp"
+#include "ch_ilu.hpp"
 #include "mlp.hpp"
 #include "ops.hpp"
 
@@ -81,6 +82,8 @@ struct hps_operator_s
   int                  precond_kind;
   double              *diag, *inv_diag64, *blocks, *inv_blocks64;
   float               *inv_diag32, *inv_blocks32;
+  double              *ch64;     // ILU(0) factors of the Cahn-Hilliard block
+  float               *ch32;
   int                 *d_singular;
   // kernel timing: a ring of `tcap` event triples (before the main kernel,
   // between it and the fix-up, after the fix-up); tcount calls recorded
@@ -322,6 +325,8 @@ extern "C" int hps_destroy(hps_operator op)
   (void)hipFree(op->blocks);
   (void)hipFree(op->inv_blocks64);
   (void)hipFree(op->inv_blocks32);
+  (void)hipFree(op->ch64);
+  (void)hipFree(op->ch32);
   (void)hipFree(op->d_singular);
   (void)hipFree(op->constrained);
   (void)hipFree(op->constrained_save);
@@ -693,6 +698,34 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
   return HPS_OK;
 }
 
+// geometry, cache layout and material of the Cahn-Hilliard ILU kernels
+static hps::ChParams ch_params(hps_operator op)
+{
+  hps::ChParams cp{};
+  cp.dim  = op->dim;
+  cp.nc[0] = op->p.n_cells[0];
+  cp.nc[1] = op->p.n_cells[1];
+  cp.nc[2] = op->dim == 3 ? op->p.n_cells[2] : 1;
+  for (int d = 0; d < 3; ++d)
+    {
+      cp.nn[d]    = d < op->dim ? cp.nc[d] + 1 : 1;
+      cp.nt[d]    = op->kp.nt[d];
+      cp.inv_h[d] = op->kp.inv_h[d];
+    }
+  cp.n_nodes  = op->n_nodes;
+  cp.n_comp   = op->nc;
+  cp.G        = op->n_grains;
+  cp.tx       = op->ops->tx;
+  cp.ty       = op->ops->ty;
+  cp.tz       = op->kp.tz;
+  cp.nt_lanes = op->ops->nt;
+  cp.nf       = op->ops->n_fields;
+  cp.jxw      = op->kp.jxw;
+  cp.pp       = op->kp.pp;
+  hps::ch_layout(cp);
+  return cp;
+}
+
 // diagonal / node blocks assembled by probing (FP64 cache and arithmetic,
 // or FP32 for the FP32 inner solve), then inverted in FP64
 static int precond_update(hps_operator op, int kind, bool fp32)
@@ -703,6 +736,30 @@ static int precond_update(hps_operator op, int kind, bool fp32)
     return fail(HPS_ERR_NOT_INITIALIZED, "preconditioner update before hps_set_linearization");
   const int64_t n = op->n_dofs, nn = op->n_nodes;
   const int     nc = op->nc;
+  if (kind == HPS_PRECON_BLOCK_CH_ILU)
+    {
+      if (op->q2 || op->tens)
+        return fail(HPS_ERR_UNSUPPORTED,
+                    "the Cahn-Hilliard ILU preconditioner is built for Q1, scalar mobility");
+      if (op->n_constrained)
+        return fail(HPS_ERR_UNSUPPORTED,
+                    "the Cahn-Hilliard ILU preconditioner takes no constrained nodes");
+      // the Allen-Cahn block: the point-Jacobi inverse diagonal
+      if (int r = precond_update(op, HPS_PRECON_POINT_JACOBI, fp32))
+        return r;
+      const int64_t nA = nn * (op->dim == 3 ? 27 : 9) * 4;
+      if (!op->ch64)
+        {
+          HPS_HIP(hipMalloc(&op->ch64, nA * sizeof(double)));
+          HPS_HIP(hipMalloc(&op->ch32, nA * sizeof(float)));
+        }
+      hps::ch_assemble(ch_params(op), op->cache64, op->ch64, op->stream);
+      HPS_LAUNCHED();
+      hps::ch_factor(ch_params(op), op->ch64, op->ch32, op->stream);
+      HPS_LAUNCHED();
+      op->precond_kind = kind;
+      return HPS_OK;
+    }
   if (kind == HPS_PRECON_POINT_JACOBI)
     {
       if (!op->diag)
@@ -818,7 +875,9 @@ extern "C" int hps_precond_apply_f64(hps_operator op, double *dst, const double 
 {
   if (int r = check_op(op))
     return r;
-  if (op->precond_kind == HPS_PRECON_POINT_JACOBI)
+  if (op->precond_kind == HPS_PRECON_BLOCK_CH_ILU)
+    hps::ch_apply<double>(ch_params(op), op->ch64, op->inv_diag64, src, dst, op->stream);
+  else if (op->precond_kind == HPS_PRECON_POINT_JACOBI)
     hps::diag_apply64(op->stream, op->n_dofs, op->inv_diag64, src, dst);
   else if (op->precond_kind == HPS_PRECON_NODE_BLOCK_JACOBI)
     hps::block_apply64(op->stream, op->n_nodes, op->nc, op->inv_blocks64, src, dst);
@@ -832,7 +891,9 @@ extern "C" int hps_precond_apply_f32(hps_operator op, float *dst, const float *s
 {
   if (int r = check_op(op))
     return r;
-  if (op->precond_kind == HPS_PRECON_POINT_JACOBI)
+  if (op->precond_kind == HPS_PRECON_BLOCK_CH_ILU)
+    hps::ch_apply<float>(ch_params(op), op->ch32, op->inv_diag32, src, dst, op->stream);
+  else if (op->precond_kind == HPS_PRECON_POINT_JACOBI)
     hps::diag_apply32(op->stream, op->n_dofs, op->inv_diag32, src, dst);
   else if (op->precond_kind == HPS_PRECON_NODE_BLOCK_JACOBI)
     hps::block_apply32(op->stream, op->n_nodes, op->nc, op->inv_blocks32, src, dst);

@@ -331,11 +331,13 @@ class SinteringOperatorGeneric:
 
 class Preconditioner:
     """PreconditionerBase (I/lac/preconditioners.h:77-112) over the
-    operator's Jacobi data: "InverseDiagonalMatrix" or
-    "InverseComponentBlockDiagonalMatrix"."""
+    operator's data: "InverseDiagonalMatrix", "InverseComponentBlockDiagonalMatrix"
+    or "BlockPreconditioner2" (ILU(0) of the Cahn-Hilliard block in parity-
+    colour order, inverse diagonal on the order parameters)."""
 
     KINDS = {"InverseDiagonalMatrix": _ffi.HPS_PRECON_POINT_JACOBI,
-             "InverseComponentBlockDiagonalMatrix": _ffi.HPS_PRECON_NODE_BLOCK_JACOBI}
+             "InverseComponentBlockDiagonalMatrix": _ffi.HPS_PRECON_NODE_BLOCK_JACOBI,
+             "BlockPreconditioner2": _ffi.HPS_PRECON_BLOCK_CH_ILU}
 
     def __init__(self, op: SinteringOperatorGeneric, kind="InverseDiagonalMatrix"):
         self.op = op

@@ -23,7 +23,8 @@ from .operator import SinteringOperatorGeneric, TimeIntegratorData
 
 class NewtonSolver:
     PRECONDITIONERS = {"InverseDiagonalMatrix": _ffi.HPS_PRECON_POINT_JACOBI,
-                       "InverseComponentBlockDiagonalMatrix": _ffi.HPS_PRECON_NODE_BLOCK_JACOBI}
+                       "InverseComponentBlockDiagonalMatrix": _ffi.HPS_PRECON_NODE_BLOCK_JACOBI,
+                       "BlockPreconditioner2": _ffi.HPS_PRECON_BLOCK_CH_ILU}
 
     def __init__(self, op: SinteringOperatorGeneric, nl_rel_tol=1e-5, nl_abs_tol=1e-20,
                  nl_max_iter=10, l_rel_tol=1e-2, l_abs_tol=1e-10, l_max_iter=1000,

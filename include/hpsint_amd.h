@@ -54,7 +54,10 @@ enum
 enum
 {
   HPS_PRECON_POINT_JACOBI      = 0, /* InverseDiagonalMatrix, I/lac/preconditioners.h:160-215 */
-  HPS_PRECON_NODE_BLOCK_JACOBI = 1  /* InverseComponentBlockDiagonalMatrix, :218-318 */
+  HPS_PRECON_NODE_BLOCK_JACOBI = 1, /* InverseComponentBlockDiagonalMatrix, :218-318 */
+  HPS_PRECON_BLOCK_CH_ILU      = 2  /* BlockPreconditioner2 with its default sub-preconditioners
+                                       (ILU on the Cahn-Hilliard block, InverseDiagonalMatrix on
+                                       the Allen-Cahn block), S/preconditioners.h:983-1282 */
 };
 
 typedef struct hps_operator_s *hps_operator;
@@ -154,7 +157,13 @@ int hps_vmult_f32(hps_operator op, float *dst, const float *src);
  * the inverse diagonal as OperatorBase::compute_inverse_diagonal does
  * (S/operator_base.h:364-389: 1/d if |d| > 1e-10 else 1);
  * HPS_PRECON_NODE_BLOCK_JACOBI inverts every node's n_c x n_c block
- * (InverseComponentBlockDiagonalMatrix::do_update, Gauss-Jordan). */
+ * (InverseComponentBlockDiagonalMatrix::do_update, Gauss-Jordan);
+ * HPS_PRECON_BLOCK_CH_ILU assembles the Cahn-Hilliard block (the tangent's
+ * c and mu rows and columns) from the linearisation cache, factors it by
+ * ILU(0) with the nodes in 2^dim parity colours (each colour a parallel
+ * stage; c before mu within a node), and takes the inverse diagonal on the
+ * order parameters (Q1, scalar mobility, no constrained nodes; the
+ * assembly and factors are FP64, the FP32 apply uses an FP32 copy). */
 int hps_precond_update(hps_operator op, int kind);
 /* The same setup for the FP32 inner Krylov solve: the diagonal / blocks are
  * assembled from the FP32 cache in FP32 arithmetic, then inverted in FP64

@@ -24,10 +24,11 @@ import scipy.sparse.linalg as spla
 import pyoracle as po
 
 
-def sparse_tangent(p, cache):
+def sparse_tangent(p, cache, keep_zeros=False):
     """the tangent as a CSR matrix (node-interleaved numbering), assembled
     from oracle vmults of coloured probe vectors: nodes 3 apart along every
-    direction have disjoint Q1 neighbourhoods"""
+    direction have disjoint Q1 neighbourhoods.  keep_zeros: store the whole
+    Q1 pattern (every neighbour pair), entries that vanish included"""
     assert p.degree == 1
     nc, dim = p.n_comp, p.dim
     nn = [p.n_cells[d] + 1 for d in range(dim)]
@@ -65,7 +66,7 @@ def sparse_tangent(p, cache):
             rr = np.nonzero(owner >= 0)[0]
             for a in range(nc):
                 v = y[rr, a]
-                nz = v != 0.0
+                nz = np.ones(v.shape, dtype=bool) if keep_zeros else v != 0.0
                 rows.append(rr[nz] * nc + a)
                 cols.append(owner[rr[nz]] * nc + c)
                 vals.append(v[nz])
@@ -151,3 +152,33 @@ def study(p, u, old, seed=0):
         out[name] = (gmres_iterations(A, M, b, rtol=1e-2),
                      gmres_iterations(A, M, z, rtol=1e-6))
     return out
+
+
+def parity_colour_ilu(A, nc, nn_dim):
+    """the GPU's BlockPreconditioner2 restated: ILU(0) of the Cahn-Hilliard
+    block (rows and columns of c and mu, the whole Q1 pattern) with the
+    nodes ordered by parity colour (x&1 | (y&1)<<1 | (z&1)<<2) and index,
+    c before mu; inverse diagonal (|d| <= 1e-10 -> 1) on the order
+    parameters.  Returns the apply function."""
+    n = A.shape[0]
+    nn = n // nc
+    node = np.arange(nn)
+    colour = np.zeros(nn, dtype=np.int64)
+    r = node.copy()
+    for d, m in enumerate(nn_dim):
+        colour |= (r % m) % 2 << d
+        r //= m
+    order = np.argsort(colour * nn + node, kind="stable")
+    ch = np.concatenate([[k * nc, k * nc + 1] for k in order])
+    Ach = A.tocsr()[ch][:, ch].tocsr()
+    ilu = po.ILU0(Ach)
+    d = A.diagonal()
+    inv_d = np.where(np.abs(d) > 1e-10, 1.0 / d, 1.0)
+    ac = np.setdiff1d(np.arange(n), ch)
+
+    def apply(v):
+        out = np.empty_like(v)
+        out[ch] = ilu.solve(v[ch])
+        out[ac] = inv_d[ac] * v[ac]
+        return out
+    return apply
