@@ -48,6 +48,11 @@ RESIDUAL_FLOP_128 = 8.25e9
 FP64_PEAK_TFLOPS = 78.6
 
 
+# the reference's default block preconditioner (ILU(0) on the Cahn-Hilliard
+# block, inverse diagonal on the Allen-Cahn block)
+BP2 = "BlockPreconditioner2"
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,7 +167,11 @@ def predictor_study(args):
       mp_euler      FP32 inner Krylov + the reference's Euler predictor
                     (S/driver.h:2005-2011);
       mp64_learned  FP32 tangent and preconditioner with an FP64 Krylov basis
-                    + learned predictor."""
+                    + learned predictor;
+    and, with the reference's default block preconditioner
+    (BlockPreconditioner2: ILU(0) of the Cahn-Hilliard block, inverse
+    diagonal of the Allen-Cahn block) in place of the node-block Jacobi:
+      fp64_bp2, mp_bp2, mp_learned_bp2."""
     import torch
     from hpsint_amd import (CartesianMesh, FreeEnergyParams, MobilityParams,
                             SinteringOperatorGeneric, packing)
@@ -199,17 +208,16 @@ def predictor_study(args):
     torch.cuda.empty_cache()
     held = collect_trajectory(op, fields(args.study_seeds), n_steps=3, dt=dt, nl_rel_tol=1e-8)
 
-    def run(fp32, predictor):
-        solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
-                              preconditioner="InverseComponentBlockDiagonalMatrix",
+    def run(fp32, predictor, prec="InverseComponentBlockDiagonalMatrix"):
+        solver = NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2, preconditioner=prec,
                               inner_fp32=fp32)
         s = TimeStepper(op, solver, order=1, dt=dt, predictor=predictor)
         s.initialize(held[3])
         s.history = [held[3].clone(), held[2].clone()]
         s.t = 3 * dt
         rs = [s.make_step() for _ in range(args.timesteps)]
-        progress("predictor: held-out steps done (inner_fp32 %s, predictor %s)"
-                 % (fp32, type(predictor).__name__ if predictor else None))
+        progress("predictor: held-out steps done (inner_fp32 %s, predictor %s, %s)"
+                 % (fp32, type(predictor).__name__ if predictor else None, prec))
         return summary(rs)
 
     euler = ExtrapolationPredictor(op, "Euler")
@@ -221,11 +229,17 @@ def predictor_study(args):
            "fp64_plain": run(0, None), "mp_learned": run(1, pred), "mp_only": run(1, None),
            "fp64_learned": run(0, pred), "mp_euler": run(1, euler),
            "mp64_learned": run(2, pred),
+           "fp64_bp2": run(0, None, BP2), "mp_bp2": run(1, None, BP2),
+           "mp_learned_bp2": run(1, pred, BP2),
            "trajectory_s": t_traj, "predictor_training_s": t_fit,
            "predictor_final_loss": pred.losses[-1]}
     base = out["fp64_plain"]["s_per_step"]
-    for k in ("mp_learned", "mp_only", "fp64_learned", "mp_euler", "mp64_learned"):
+    for k in ("mp_learned", "mp_only", "fp64_learned", "mp_euler", "mp64_learned",
+              "fp64_bp2", "mp_bp2", "mp_learned_bp2"):
         out["speedup_%s_vs_fp64" % k] = base / out[k]["s_per_step"]
+    # both sides with the reference's default block preconditioner
+    out["speedup_mp_learned_bp2_vs_fp64_bp2"] = (out["fp64_bp2"]["s_per_step"]
+                                                 / out["mp_learned_bp2"]["s_per_step"])
     del op, held
     torch.cuda.empty_cache()
     return out
@@ -313,17 +327,22 @@ def mp_workload(args):
         first.make_step()
         u1 = first.u.clone()
         del first
-        res = {"workload": "BDF1 dt=%g, Newton rtol 1e-8, GMRES rtol 1e-2, node-block Jacobi; "
-                           "one untimed step, then %d timed steps per method"
+        res = {"workload": "BDF1 dt=%g, Newton rtol 1e-8, GMRES rtol 1e-2, node-block Jacobi "
+                           "(*_bp2: BlockPreconditioner2); one untimed step, then %d timed "
+                           "steps per method"
                            % (dt, args.mp_timesteps)}
-        for name, fp32, tol in (("fp64_plain", 0, 0.0), ("mp_only", 1, 0.0),
-                                ("mp64_only", 2, 0.0), ("fp64_cut", 0, args.cut_tol),
-                                ("mp_cut", 1, args.cut_tol)):
+        nb = "InverseComponentBlockDiagonalMatrix"
+        for name, fp32, tol, prec in (("fp64_plain", 0, 0.0, nb), ("mp_only", 1, 0.0, nb),
+                                      ("mp64_only", 2, 0.0, nb),
+                                      ("fp64_cut", 0, args.cut_tol, nb),
+                                      ("mp_cut", 1, args.cut_tol, nb),
+                                      ("fp64_bp2", 0, 0.0, BP2), ("mp_bp2", 1, 0.0, BP2)):
             if tol <= 0.0 and name.endswith("cut"):
                 continue
             op.set_grain_cut_off(tol)
             s = TimeStepper(op, NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
-                                             inner_fp32=fp32), order=1, dt=dt)
+                                             inner_fp32=fp32, preconditioner=prec),
+                            order=1, dt=dt)
             s.initialize(u1)
             s.history = [u1.clone(), u.clone()]
             s.t = dt
@@ -333,9 +352,11 @@ def mp_workload(args):
             progress("27 spheres: %s steps done" % name)
         op.set_grain_cut_off(0.0)
         base = res["fp64_plain"]["s_per_step"]
-        for name in ("mp_only", "mp64_only", "fp64_cut", "mp_cut"):
+        for name in ("mp_only", "mp64_only", "fp64_cut", "mp_cut", "fp64_bp2", "mp_bp2"):
             if name in res:
                 res["speedup_%s_vs_fp64" % name] = base / res[name]["s_per_step"]
+        res["speedup_mp_bp2_vs_fp64_bp2"] = (res["fp64_bp2"]["s_per_step"]
+                                             / res["mp_bp2"]["s_per_step"])
         out["timesteps"] = res
     del op, u
     torch.cuda.empty_cache()
@@ -426,7 +447,8 @@ def cfg1_timesteps(op, u0, dt, args):
     """The headline workload's timesteps (BDF1, Newton rtol 1e-8, GMRES rtol
     1e-2, node-block Jacobi), after 3 untimed plain steps: plain FP64, FP32
     inner Krylov, and FP32 inner Krylov with the reference's Euler
-    predictor"""
+    predictor; then plain FP64 and FP32 inner Krylov with the reference's
+    default block preconditioner (BlockPreconditioner2)"""
     import torch
     from hpsint_amd.solvers import ExtrapolationPredictor, NewtonSolver, TimeStepper
     s = TimeStepper(op, NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2), order=1, dt=dt)
@@ -437,10 +459,13 @@ def cfg1_timesteps(op, u0, dt, args):
         traj.append(s.u.clone())
     del s
     res = {"workload": "3 untimed plain steps, then %d timed steps per method" % args.timesteps}
-    for name, fp32, pred in (("fp64_plain", 0, None), ("mp_only", 1, None),
-                             ("mp_euler", 1, ExtrapolationPredictor(op, "Euler"))):
+    nb = "InverseComponentBlockDiagonalMatrix"
+    for name, fp32, pred, prec in (("fp64_plain", 0, None, nb), ("mp_only", 1, None, nb),
+                                   ("mp_euler", 1, ExtrapolationPredictor(op, "Euler"), nb),
+                                   ("fp64_bp2", 0, None, BP2), ("mp_bp2", 1, None, BP2)):
         st = TimeStepper(op, NewtonSolver(op, nl_rel_tol=1e-8, l_rel_tol=1e-2,
-                                          inner_fp32=fp32), order=1, dt=dt, predictor=pred)
+                                          inner_fp32=fp32, preconditioner=prec),
+                         order=1, dt=dt, predictor=pred)
         st.initialize(traj[3])
         st.history = [traj[3].clone(), traj[2].clone()]
         st.t = 3 * dt
@@ -448,6 +473,10 @@ def cfg1_timesteps(op, u0, dt, args):
     base = res["fp64_plain"]["s_per_step"]
     res["speedup_mp_only_vs_fp64"] = base / res["mp_only"]["s_per_step"]
     res["speedup_mp_euler_vs_fp64"] = base / res["mp_euler"]["s_per_step"]
+    res["speedup_fp64_bp2_vs_fp64"] = base / res["fp64_bp2"]["s_per_step"]
+    res["speedup_mp_bp2_vs_fp64"] = base / res["mp_bp2"]["s_per_step"]
+    res["speedup_mp_bp2_vs_fp64_bp2"] = (res["fp64_bp2"]["s_per_step"]
+                                         / res["mp_bp2"]["s_per_step"])
     torch.cuda.empty_cache()
     progress("128^3: steps done")
     return res

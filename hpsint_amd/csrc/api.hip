@@ -747,7 +747,8 @@ static int precond_update(hps_operator op, int kind, bool fp32)
       // the Allen-Cahn block: the point-Jacobi inverse diagonal
       if (int r = precond_update(op, HPS_PRECON_POINT_JACOBI, fp32))
         return r;
-      const int64_t nA = nn * (op->dim == 3 ? 27 : 9) * 4;
+      // the factors and the solves' work vector
+      const int64_t nA = hps::ch_storage(ch_params(op));
       if (!op->ch64)
         {
           HPS_HIP(hipMalloc(&op->ch64, nA * sizeof(double)));

@@ -8,17 +8,22 @@
 // The Cahn-Hilliard block is the tangent restricted to the rows and columns
 // of c and mu.  Q1 couples a node with its 3^DIM neighbours, so a node's row
 // is 3^DIM stencil entries of 2x2 (row c/mu, column c/mu), s = (dx+1) +
-// 3 (dy+1) [+ 9 (dz+1)]; the rows are stored colour by colour and entry by
-// entry (ChParams::off, cnt), so that the threads of one colour stage read
-// consecutive addresses.
+// 3 (dy+1) [+ 9 (dz+1)]; the rows are stored colour by colour and stencil
+// entry by stencil entry (ChParams::off, cnt), so that the threads of one
+// colour stage read consecutive 4-vectors.
 //
 // Ordering: nodes in 2^DIM colours by the parity of their coordinates
 // (colour = x&1 | (y&1)<<1 | (z&1)<<2), nodes of one colour in index order,
 // c before mu within a node.  No two neighbours share a colour, so every
 // stage of the factorisation and of the triangular solves (one colour) is a
 // parallel loop over the colour's nodes: 2^DIM launches per factorisation
-// and per solve.  ILU(0) keeps the pattern of the block; the rows are
+// and 2^(DIM+1) - 1 per solve (the last forward stage is also the first
+// backward stage).  ILU(0) keeps the pattern of the block; the rows are
 // eliminated in the IKJ order (lower entries in increasing column order).
+// The solves keep (c, mu) in a work vector in the same colour-major node
+// order, so the neighbours that a thread reads are consecutive for
+// consecutive threads, as its rows are; one more launch writes dst in node
+// order, one thread per entry (coalesced), with the Allen-Cahn components.
 //
 // The assembly forms the Cahn-Hilliard part of every cell's element matrix
 // from the linearisation cache (scalar mobility):
@@ -33,6 +38,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <utility>
 
 #include "ch_ilu.hpp"
 #include "physics.hpp"
@@ -51,55 +57,25 @@ namespace hps
       }
     };
 
-    __device__ __forceinline__ void sten_offset(int DIMv, int s, int &dx, int &dy, int &dz)
-    {
-      dx = s % 3 - 1;
-      dy = (s / 3) % 3 - 1;
-      dz = DIMv == 3 ? s / 9 - 1 : 0;
-    }
-
-    // node of colour `col` number i (x fastest among the colour's nodes)
-    __device__ __forceinline__ bool colour_node(const ChParams &cp, int col, int64_t i, int &x,
-                                                int &y, int &z)
-    {
-      const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
-      const int mx = (cp.nn[0] - px + 1) / 2, my = (cp.nn[1] - py + 1) / 2;
-      const int mz = cp.dim == 3 ? (cp.nn[2] - pz + 1) / 2 : 1;
-      if (i >= (int64_t)mx * my * mz)
-        return false;
-      x = px + 2 * (int)(i % mx);
-      y = py + 2 * (int)((i / mx) % my);
-      z = cp.dim == 3 ? pz + 2 * (int)(i / ((int64_t)mx * my)) : 0;
-      return true;
-    }
-
-    __device__ __forceinline__ bool inside(const ChParams &cp, int x, int y, int z)
-    {
-      return x >= 0 && x < cp.nn[0] && y >= 0 && y < cp.nn[1] && z >= 0 &&
-             z < (cp.dim == 3 ? cp.nn[2] : 1);
-    }
-
-    __device__ __forceinline__ int64_t node_id(const ChParams &cp, int x, int y, int z)
-    {
-      return ((int64_t)z * cp.nn[1] + y) * cp.nn[0] + x;
-    }
-
-    // the stored row of a node: entry k = 4 s + 2 a + b at base[k * stride]
+    // the stored row of a node: entry k = 4 s + 2 a + b at
+    // base[(k / 4) stride + k % 4]
     template <typename T>
     struct RowRef
     {
       T      *base;
       int64_t stride;
-      __device__ __forceinline__ T &operator[](int k) const { return base[k * stride]; }
+      __device__ __forceinline__ T &operator[](int k) const
+      {
+        return base[(k >> 2) * stride + (k & 3)];
+      }
     };
 
     template <typename T>
     __device__ __forceinline__ RowRef<T> row_of(const ChParams &cp, T *F, int x, int y, int z)
     {
       const int     col = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
-      const int64_t mx = (cp.nn[0] - (col & 1) + 1) / 2, my = (cp.nn[1] - ((col >> 1) & 1) + 1) / 2;
-      const int64_t ic = (x >> 1) + mx * ((y >> 1) + my * (z >> 1));
-      return RowRef<T>{F + cp.off[col] + ic, cp.cnt[col]};
+      const int64_t ic  = (x >> 1) + (int64_t)cp.mx[col] * ((y >> 1) + (int64_t)cp.my[col] * (z >> 1));
+      return RowRef<T>{F + cp.off[col] + 4 * ic, 4 * cp.cnt[col]};
     }
   } // namespace
 
@@ -222,164 +198,319 @@ namespace hps
       }
   }
 
-  // ------------------------------------------------------------ factorisation
-  // rows (i, c) then (i, mu) of the nodes of colour `col`, IKJ order: the
-  // lower entries (j, b) of a row in increasing column order -- the
-  // neighbours j of lower colours by (colour, index), c before mu, and for the
-  // mu row finally (i, c) -- each eliminated with the already final row
-  // (j, b): l = a / u_(j,b),(j,b); a_(i,a),(k,e) -= l u_(j,b),(k,e) for the
-  // (k, e) after (j, b) in row (j, b) that are in the pattern of row (i, a)
-  template <int DIM>
-  __global__ void __launch_bounds__(256)
-    k_ch_factor(const ChParams cp, const ChOrder ord, int col, double *__restrict__ A)
+  // a node of colour COL with in-colour index ic: its place in the colour's
+  // grid (ix, iy, iz), its coordinates and its node number
+  template <int DIM, int COL>
+  struct ColourNode
   {
-    constexpr int S = Sten<DIM>::S;
-    int           x, y, z;
-    if (!colour_node(cp, col, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, x, y, z))
-      return;
-    const RowRef<double> Ai     = row_of(cp, A, x, y, z);
-    const int            center = (S - 1) / 2;
-    for (int a = 0; a < 2; ++a)
+    int64_t ic, node;
+    int     ix, iy, iz, x, y, z;
+    __device__ __forceinline__ ColourNode(const ChParams &cp, int64_t ic_)
+      : ic(ic_)
+    {
+      const int mx = cp.mx[COL], my = cp.my[COL];
+      ix   = (int)(ic % mx);
+      iy   = (int)((ic / mx) % my);
+      iz   = DIM == 3 ? (int)(ic / ((int64_t)mx * my)) : 0;
+      x    = (COL & 1) + 2 * ix;
+      y    = ((COL >> 1) & 1) + 2 * iy;
+      z    = DIM == 3 ? ((COL >> 2) & 1) + 2 * iz : 0;
+      node = ((int64_t)z * cp.nn[1] + y) * cp.nn[0] + x;
+    }
+  };
+
+  // ------------------------------------------------------------ factorisation
+  // stencil index s <-> offset (dx, dy, dz); parity mask of the offset
+  __host__ __device__ constexpr int st_dx(int s) { return s % 3 - 1; }
+  __host__ __device__ constexpr int st_dy(int s) { return (s / 3) % 3 - 1; }
+  __host__ __device__ constexpr int st_dz(int DIM, int s) { return DIM == 3 ? s / 9 - 1 : 0; }
+  __host__ __device__ constexpr int st_mask(int DIM, int s)
+  {
+    return (st_dx(s) & 1) | ((st_dy(s) & 1) << 1) | ((st_dz(DIM, s) & 1) << 2);
+  }
+
+  // the neighbours of a node of colour COL with a lower colour, in the
+  // order of their columns: by colour, then by stencil index (the node
+  // index order within a colour); and those with a higher colour
+  template <int DIM, int COL>
+  struct FactorLists
+  {
+    int n_lower = 0, n_upper = 0;
+    int lower[26] = {}, upper[26] = {};
+    constexpr FactorLists()
+    {
+      constexpr int S = DIM == 3 ? 27 : 9;
+      for (int jc = 0; jc < COL; ++jc)
+        for (int t = 0; t < S; ++t)
+          if ((COL ^ st_mask(DIM, t)) == jc)
+            lower[n_lower++] = t;
+      for (int t = 0; t < S; ++t)
+        if ((COL ^ st_mask(DIM, t)) > COL)
+          upper[n_upper++] = t;
+    }
+  };
+
+  // row (i, AR) (AR = 0: c, 1: mu) of the nodes of colour COL, held in
+  // registers: R[s] = the entries (AR, c) and (AR, mu) of stencil entry s.
+  // IKJ order: the lower entries (j, b) in increasing column order -- the
+  // neighbours j of lower colours by (colour, index), c before mu, and for
+  // the mu row finally (i, c) -- each eliminated with the already final row
+  // (j, b): l = a / u_(j,b),(j,b); a_(i,AR),(k,e) -= l u_(j,b),(k,e) for
+  // the (k, e) after (j, b) in row (j, b) that are in the pattern of row
+  // (i, AR): (j, mu) when b = c, and the neighbours k of j of a higher
+  // colour than j's that are neighbours of i (or i itself).  Entries of
+  // neighbours outside the domain are zero in the assembled block and stay
+  // zero.  The final row is written in FP64 and FP32.
+  template <int DIM, int COL, int AR>
+  __device__ __forceinline__ void ch_factor_row(const ChParams &cp, const ColourNode<DIM, COL> &n,
+                                                double *__restrict__ A, float *__restrict__ A32)
+  {
+    constexpr int                   S      = DIM == 3 ? 27 : 9;
+    constexpr int                   center = (S - 1) / 2;
+    constexpr FactorLists<DIM, COL> fl{};
+    constexpr int px = COL & 1, py = (COL >> 1) & 1, pz = (COL >> 2) & 1;
+    double2      *Ai = reinterpret_cast<double2 *>(A + cp.off[COL]) + 2 * n.ic + AR;
+    const int64_t st = 2 * cp.cnt[COL];  // double2 stride between stencil entries
+    double        R[S][2];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
       {
-        for (int t = 0; t < ord.n_lower[col]; ++t)
+        const double2 v = Ai[s * st];
+        R[s][0]         = v.x;
+        R[s][1]         = v.y;
+      }
+#pragma unroll
+    for (int t = 0; t < fl.n_lower; ++t)
+      {
+        const int  tt = fl.lower[t];
+        const int  dx = st_dx(tt), dy = st_dy(tt), dz = st_dz(DIM, tt);
+        const int  jc = COL ^ st_mask(DIM, tt);
+        const bool ok = (dx >= 0 || n.x > 0) && (dx <= 0 || n.x + 1 < cp.nn[0]) &&
+                        (dy >= 0 || n.y > 0) && (dy <= 0 || n.y + 1 < cp.nn[1]) &&
+                        (dz >= 0 || n.z > 0) && (dz <= 0 || n.z + 1 < cp.nn[2]);
+        if (!ok)
+          continue;
+        const int      jx = n.ix + ((px + dx) >> 1), jy = n.iy + ((py + dy) >> 1);
+        const int      jz = n.iz + ((pz + dz) >> 1);
+        const int64_t  j  = jx + (int64_t)cp.mx[jc] * (jy + (int64_t)cp.my[jc] * jz);
+        const double4 *Aj = reinterpret_cast<const double4 *>(A + cp.off[jc]) + j;
+        const double4  uc = Aj[center * cp.cnt[jc]];
+        // (j, c), then (j, mu) with the entry that (j, c) updated
+        const double l0 = R[tt][0] / uc.x;
+        R[tt][0]        = l0;
+        R[tt][1] -= l0 * uc.y;
+        const double l1 = R[tt][1] / uc.w;
+        R[tt][1]        = l1;
+#pragma unroll
+        for (int u = 0; u < S; ++u)
           {
-            const int s = ord.order[col][t];
-            int       dx, dy, dz;
-            sten_offset(DIM, s, dx, dy, dz);
-            if (!inside(cp, x + dx, y + dy, z + dz))
+            if (u == center || (jc ^ st_mask(DIM, u)) <= jc)
               continue;
-            const RowRef<double> Aj = row_of(cp, A, x + dx, y + dy, z + dz);
-            const int            jc = col ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
-            for (int b = 0; b < 2; ++b)
-              {
-                const double l = Ai[s * 4 + 2 * a + b] / Aj[center * 4 + 3 * b];
-                Ai[s * 4 + 2 * a + b] = l;
-                // row (j, b) after (j, b): (j, mu) when b = c, then the
-                // neighbours of j of higher colour than jc; those that are
-                // in row (i, a)'s pattern: i itself and i's neighbours
-                if (b == 0)
-                  Ai[s * 4 + 2 * a + 1] -= l * Aj[center * 4 + 1];
-                for (int u = 0; u < S; ++u)
-                  {
-                    int ex, ey, ez;
-                    sten_offset(DIM, u, ex, ey, ez);
-                    if (u == center)
-                      continue;
-                    const int kc = jc ^ ((ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2));
-                    if (kc <= jc)
-                      continue;
-                    // k = j + e relative to i: (dx + ex, ...)
-                    const int fx = dx + ex, fy = dy + ey, fz = dz + ez;
-                    if (fx < -1 || fx > 1 || fy < -1 || fy > 1 || fz < -1 || fz > 1)
-                      continue;
-                    if (!inside(cp, x + fx, y + fy, z + fz))
-                      continue;
-                    const int v = DIM == 3 ? (fx + 1) + 3 * (fy + 1) + 9 * (fz + 1)
-                                           : (fx + 1) + 3 * (fy + 1);
-                    Ai[v * 4 + 2 * a + 0] -= l * Aj[u * 4 + 2 * b + 0];
-                    Ai[v * 4 + 2 * a + 1] -= l * Aj[u * 4 + 2 * b + 1];
-                  }
-              }
+            const int fx = dx + st_dx(u), fy = dy + st_dy(u), fz = dz + st_dz(DIM, u);
+            if (fx < -1 || fx > 1 || fy < -1 || fy > 1 || fz < -1 || fz > 1)
+              continue;
+            const int     v  = DIM == 3 ? (fx + 1) + 3 * (fy + 1) + 9 * (fz + 1)
+                                        : (fx + 1) + 3 * (fy + 1);
+            const double4 uu = Aj[u * cp.cnt[jc]];
+            R[v][0] -= l0 * uu.x;
+            R[v][1] -= l0 * uu.y;
+            R[v][0] -= l1 * uu.z;
+            R[v][1] -= l1 * uu.w;
           }
-        if (a == 1)
+      }
+    if (AR == 1)
+      {
+        // the lower entry (i, c) of row (i, mu), with the final row (i, c)
+        const double2 *Ac = Ai - 1;
+        const double2  cc = Ac[center * st];
+        const double   l  = R[center][0] / cc.x;
+        R[center][0]      = l;
+        R[center][1] -= l * cc.y;
+#pragma unroll
+        for (int t = 0; t < fl.n_upper; ++t)
           {
-            // the lower entry (i, c) of row (i, mu), with the final row (i, c)
-            const double l = Ai[center * 4 + 2] / Ai[center * 4 + 0];
-            Ai[center * 4 + 2] = l;
-            Ai[center * 4 + 3] -= l * Ai[center * 4 + 1];
-            for (int u = 0; u < S; ++u)
-              {
-                int ex, ey, ez;
-                sten_offset(DIM, u, ex, ey, ez);
-                if (u == center || !inside(cp, x + ex, y + ey, z + ez))
-                  continue;
-                const int kc = col ^ ((ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2));
-                if (kc <= col)
-                  continue;
-                Ai[u * 4 + 2] -= l * Ai[u * 4 + 0];
-                Ai[u * 4 + 3] -= l * Ai[u * 4 + 1];
-              }
+            const double2 cu = Ac[fl.upper[t] * st];
+            R[fl.upper[t]][0] -= l * cu.x;
+            R[fl.upper[t]][1] -= l * cu.y;
           }
+      }
+    float2 *Ai32 = reinterpret_cast<float2 *>(A32 + cp.off[COL]) + 2 * n.ic + AR;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      {
+        Ai[s * st]   = double2{R[s][0], R[s][1]};
+        Ai32[s * st] = float2{(float)R[s][0], (float)R[s][1]};
       }
   }
 
-  template <int DIM>
-  __global__ void k_ch_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b)
+  template <int DIM, int COL>
+  __global__ void __launch_bounds__(256)
+    k_ch_factor(const ChParams cp, double *__restrict__ A, float *__restrict__ A32)
   {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-      b[i] = (float)a[i];
+    const int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= cp.cnt[COL])
+      return;
+    const ColourNode<DIM, COL> n(cp, ic);
+    ch_factor_row<DIM, COL, 0>(cp, n, A, A32);
+    ch_factor_row<DIM, COL, 1>(cp, n, A, A32);
   }
 
   // ------------------------------------------------------------------ solves
-  // forward: y_(i,a) = r_(i,a) - sum_lower L y; the (c, mu) components of
-  // the interleaved vectors (n_comp per node)
-  template <int DIM, typename T>
-  __global__ void __launch_bounds__(256)
-    k_ch_forward(const ChParams cp, int col, const T *__restrict__ F, const T *__restrict__ r,
-                 T *__restrict__ y)
+  template <typename T>
+  struct Vec;
+  template <>
+  struct Vec<float>
   {
-    constexpr int S = Sten<DIM>::S;
-    int           x, y0, z;
-    if (!colour_node(cp, col, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, x, y0, z))
-      return;
-    const int64_t   i  = node_id(cp, x, y0, z);
-    const RowRef<const T> Fi = row_of(cp, F, x, y0, z);
-    const int       nc = cp.n_comp;
-    T             s0 = r[i * nc + 0], s1 = r[i * nc + 1];
-    for (int s = 0; s < S; ++s)
+    using V2 = float2;
+    using V4 = float4;
+  };
+  template <>
+  struct Vec<double>
+  {
+    using V2 = double2;
+    using V4 = double4;
+  };
+
+  // the stencil entries of a node of colour COL whose neighbour has a lower
+  // (LOWER) or a higher colour, in stencil order
+  template <int DIM, int COL, bool LOWER>
+  struct Neighbours
+  {
+    int n     = 0;
+    int s[26] = {};
+    constexpr Neighbours()
+    {
+      for (int t = 0; t < (DIM == 3 ? 27 : 9); ++t)
+        {
+          const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = DIM == 3 ? t / 9 - 1 : 0;
+          const int jc = COL ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+          if (LOWER ? jc < COL : jc > COL)
+            s[n++] = t;
+        }
+    }
+  };
+
+  // (s0, s1) -= sum over the lower (LOWER) or upper neighbours j of the
+  // node's 2x2 entry times the work vector's (c, mu) at j
+  template <int DIM, int COL, bool LOWER, typename T>
+  __device__ __forceinline__ void ch_gather(const ChParams &cp, const T *__restrict__ F,
+                                            const T *__restrict__ W, const ColourNode<DIM, COL> &n,
+                                            T &s0, T &s1)
+  {
+    using V2 = typename Vec<T>::V2;
+    using V4 = typename Vec<T>::V4;
+    constexpr Neighbours<DIM, COL, LOWER> nb{};
+    constexpr int px = COL & 1, py = (COL >> 1) & 1, pz = (COL >> 2) & 1;
+    const V4     *Fr = reinterpret_cast<const V4 *>(F + cp.off[COL]) + n.ic;
+    const V2     *W2 = reinterpret_cast<const V2 *>(W);
+#pragma unroll
+    for (int t = 0; t < nb.n; ++t)
       {
-        int dx, dy, dz;
-        sten_offset(DIM, s, dx, dy, dz);
-        const int jc = col ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
-        if (jc >= col || !inside(cp, x + dx, y0 + dy, z + dz))
-          continue;
-        const int64_t j  = node_id(cp, x + dx, y0 + dy, z + dz);
-        const T       yc = y[j * nc + 0], ym = y[j * nc + 1];
-        s0 -= Fi[s * 4 + 0] * yc + Fi[s * 4 + 1] * ym;
-        s1 -= Fi[s * 4 + 2] * yc + Fi[s * 4 + 3] * ym;
+        const int sx = nb.s[t];
+        const int dx = sx % 3 - 1, dy = (sx / 3) % 3 - 1, dz = DIM == 3 ? sx / 9 - 1 : 0;
+        const int jc = COL ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+        const bool ok = (dx >= 0 || n.x > 0) && (dx <= 0 || n.x + 1 < cp.nn[0]) &&
+                        (dy >= 0 || n.y > 0) && (dy <= 0 || n.y + 1 < cp.nn[1]) &&
+                        (dz >= 0 || n.z > 0) && (dz <= 0 || n.z + 1 < cp.nn[2]);
+        // the neighbour's place in its colour's grid: (x + dx) / 2 etc.;
+        // outside the domain the loads read the colour's first node and the
+        // select drops the term (the loads of all neighbours stay
+        // unconditional, so they issue together)
+        const int     jx = n.ix + ((px + dx) >> 1), jy = n.iy + ((py + dy) >> 1);
+        const int     jz = n.iz + ((pz + dz) >> 1);
+        const int64_t j  = ok ? jx + (int64_t)cp.mx[jc] * (jy + (int64_t)cp.my[jc] * jz) : 0;
+        const V2      w  = W2[cp.noff[jc] + j];
+        const V4      f  = Fr[sx * cp.cnt[COL]];
+        const T       t0 = f.x * w.x + f.y * w.y, t1 = f.z * w.x + f.w * w.y;
+        s0 -= ok ? t0 : T(0);
+        s1 -= ok ? t1 : T(0);
       }
-    const int center = (S - 1) / 2;
-    s1 -= Fi[center * 4 + 2] * s0;
-    y[i * nc + 0] = s0;
-    y[i * nc + 1] = s1;
   }
 
-  // backward: x_(i,a) = (y_(i,a) - sum_upper U x) / u_(i,a),(i,a), mu first;
-  // the Allen-Cahn components of the node get the inverse diagonal (in the
-  // colour pass of their node)
+  // forward stage of colour COL: y_(i,c) = r_(i,c) - sum_lower L y,
+  // y_(i,mu) = r_(i,mu) - sum_lower L y - l_(mu,c) y_(i,c), into the work
+  // vector.  The last colour has no upper neighbours, so its stage also
+  // finishes the backward solve of its nodes:
+  // x_mu = y_mu / u_(mu,mu), x_c = (y_c - u_(c,mu) x_mu) / u_(c,c)
+  template <int DIM, int COL, typename T>
+  __global__ void __launch_bounds__(256)
+    k_ch_forward(const ChParams cp, const T *__restrict__ F, const T *__restrict__ src,
+                 T *__restrict__ W)
+  {
+    using V2            = typename Vec<T>::V2;
+    using V4            = typename Vec<T>::V4;
+    constexpr bool LAST = COL == (1 << DIM) - 1;
+    const int64_t  ic   = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= cp.cnt[COL])
+      return;
+    const ColourNode<DIM, COL> n(cp, ic);
+    T                          s0 = src[n.node * cp.n_comp + 0];
+    T                          s1 = src[n.node * cp.n_comp + 1];
+    ch_gather<DIM, COL, true>(cp, F, W, n, s0, s1);
+    constexpr int center = DIM == 3 ? 13 : 4;
+    const V4      u      = reinterpret_cast<const V4 *>(F + cp.off[COL])[center * cp.cnt[COL] + ic];
+    s1 -= u.z * s0;
+    if (LAST)
+      {
+        s1 = s1 / u.w;
+        s0 = (s0 - u.y * s1) / u.x;
+      }
+    reinterpret_cast<V2 *>(W)[cp.noff[COL] + ic] = V2{s0, s1};
+  }
+
+  // backward stage of colour COL (all but the last): x_(i,a) from the work
+  // vector's y_(i,a) minus the upper entries times the final x of the
+  // higher colours, x_mu first, into the work vector
+  template <int DIM, int COL, typename T>
+  __global__ void __launch_bounds__(256)
+    k_ch_backward(const ChParams cp, const T *__restrict__ F, T *__restrict__ W)
+  {
+    using V2         = typename Vec<T>::V2;
+    using V4         = typename Vec<T>::V4;
+    const int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= cp.cnt[COL])
+      return;
+    const ColourNode<DIM, COL> n(cp, ic);
+    V2                        *W2 = reinterpret_cast<V2 *>(W);
+    const V2                   yv = W2[cp.noff[COL] + ic];
+    T                          s0 = yv.x, s1 = yv.y;
+    ch_gather<DIM, COL, false>(cp, F, W, n, s0, s1);
+    constexpr int center = DIM == 3 ? 13 : 4;
+    const V4      u      = reinterpret_cast<const V4 *>(F + cp.off[COL])[center * cp.cnt[COL] + ic];
+    const T       xm     = s1 / u.w;
+    const T       xc     = (s0 - u.y * xm) / u.x;
+    W2[cp.noff[COL] + ic] = V2{xc, xm};
+  }
+
+  // dst in node order, one thread per entry: (c, mu) from the work vector
+  // (the node's colour and place in its colour's grid), the Allen-Cahn
+  // components inv_diag * src.  Workgroup (row, chunk): blockIdx.x is the
+  // node row (y, z), blockIdx.y the chunk of 256 entries of the row, so the
+  // index arithmetic stays in 32 bits.
   template <int DIM, typename T>
   __global__ void __launch_bounds__(256)
-    k_ch_backward(const ChParams cp, int col, const T *__restrict__ F, const T *__restrict__ inv_diag,
-                  const T *__restrict__ src, T *__restrict__ y)
+    k_ch_scatter(const ChParams cp, const T *__restrict__ W, const T *__restrict__ inv_diag,
+                 const T *__restrict__ src, T *__restrict__ dst)
   {
-    constexpr int S = Sten<DIM>::S;
-    int           x, y0, z;
-    if (!colour_node(cp, col, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, x, y0, z))
+    const int nc  = cp.n_comp;
+    const int len = cp.nn[0] * nc;
+    const int t   = (int)blockIdx.y * 256 + (int)threadIdx.x;
+    if (t >= len)
       return;
-    const int64_t   i  = node_id(cp, x, y0, z);
-    const RowRef<const T> Fi = row_of(cp, F, x, y0, z);
-    const int       nc = cp.n_comp;
-    T             s0 = y[i * nc + 0], s1 = y[i * nc + 1];
-    for (int s = 0; s < S; ++s)
+    const int     row = (int)blockIdx.x;
+    const int64_t i   = (int64_t)row * len + t;
+    const int     x   = t / nc;
+    const int     c   = t - x * nc;
+    if (c >= 2)
       {
-        int dx, dy, dz;
-        sten_offset(DIM, s, dx, dy, dz);
-        const int jc = col ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
-        if (jc <= col || !inside(cp, x + dx, y0 + dy, z + dz))
-          continue;
-        const int64_t j  = node_id(cp, x + dx, y0 + dy, z + dz);
-        const T       xc = y[j * nc + 0], xm = y[j * nc + 1];
-        s0 -= Fi[s * 4 + 0] * xc + Fi[s * 4 + 1] * xm;
-        s1 -= Fi[s * 4 + 2] * xc + Fi[s * 4 + 3] * xm;
+        dst[i] = inv_diag[i] * src[i];
+        return;
       }
-    const int center = (S - 1) / 2;
-    const T   xm     = s1 / Fi[center * 4 + 3];
-    const T   xc     = (s0 - Fi[center * 4 + 1] * xm) / Fi[center * 4 + 0];
-    y[i * nc + 0]    = xc;
-    y[i * nc + 1]    = xm;
-    for (int c = 2; c < nc; ++c)
-      y[i * nc + c] = inv_diag[i * nc + c] * src[i * nc + c];
+    const int     y   = row % cp.nn[1];
+    const int     z   = row / cp.nn[1];
+    const int     col = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+    const int64_t ic  = (x >> 1) + (int64_t)cp.mx[col] * ((y >> 1) + (int64_t)cp.my[col] * (z >> 1));
+    dst[i]            = W[2 * (cp.noff[col] + ic) + c];
   }
 
   // ---------------------------------------------------------------- launchers
@@ -400,36 +531,16 @@ namespace hps
   {
     const int S = cp.dim == 3 ? 27 : 9;
     cp.off[0]   = 0;
+    cp.noff[0]  = 0;
     for (int col = 0; col < 8; ++col)
       {
-        cp.cnt[col]     = col < (1 << cp.dim) ? colour_count(cp, col) : 0;
-        cp.off[col + 1] = cp.off[col] + cp.cnt[col] * S * 4;
+        const int px = col & 1, py = (col >> 1) & 1;
+        cp.mx[col]       = (cp.nn[0] - px + 1) / 2;
+        cp.my[col]       = (cp.nn[1] - py + 1) / 2;
+        cp.cnt[col]      = col < (1 << cp.dim) ? colour_count(cp, col) : 0;
+        cp.off[col + 1]  = cp.off[col] + cp.cnt[col] * S * 4;
+        cp.noff[col + 1] = cp.noff[col] + cp.cnt[col];
       }
-  }
-
-  ChOrder ch_order(int dim)
-  {
-    ChOrder o{};
-    const int S = dim == 3 ? 27 : 9, center = (S - 1) / 2, ncol = 1 << dim;
-    for (int col = 0; col < ncol; ++col)
-      {
-        // lower neighbours (colour < col) by (colour, linear offset): the
-        // linear offset orders as (dz, dy, dx) lexicographically, which is
-        // the stencil index order
-        int n = 0;
-        for (int jc = 0; jc < col; ++jc)
-          for (int s = 0; s < S; ++s)
-            {
-              if (s == center)
-                continue;
-              const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = dim == 3 ? s / 9 - 1 : 0;
-              const int m  = (dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2);
-              if ((col ^ m) == jc)
-                o.order[col][n++] = (signed char)s;
-            }
-        o.n_lower[col] = n;
-      }
-    return o;
   }
 
   void ch_assemble(const ChParams &cp, const double *cache, double *A, hipStream_t s)
@@ -451,48 +562,56 @@ namespace hps
       }
   }
 
+  template <int DIM, int... C>
+  static void ch_factor_stages(std::integer_sequence<int, C...>, const ChParams &cp, double *A,
+                               float *A32, hipStream_t s)
+  {
+    ((cp.cnt[C] ? k_ch_factor<DIM, C><<<blocks_for(cp.cnt[C]), 256, 0, s>>>(cp, A, A32) : void()),
+     ...);
+  }
+
   void ch_factor(const ChParams &cp, double *A, float *A32, hipStream_t s)
   {
-    const ChOrder ord = ch_order(cp.dim);
-    for (int col = 0; col < (1 << cp.dim); ++col)
-      {
-        const int64_t n = colour_count(cp, col);
-        if (n == 0)
-          continue;
-        if (cp.dim == 3)
-          k_ch_factor<3><<<blocks_for(n), 256, 0, s>>>(cp, ord, col, A);
-        else
-          k_ch_factor<2><<<blocks_for(n), 256, 0, s>>>(cp, ord, col, A);
-      }
-    const int64_t nA = cp.n_nodes * (cp.dim == 3 ? 27 : 9) * 4;
-    k_ch_to_f32<3><<<(unsigned)std::min<int64_t>((nA + 255) / 256, 8192), 256, 0, s>>>(nA, A, A32);
+    if (cp.dim == 3)
+      ch_factor_stages<3>(std::make_integer_sequence<int, 8>{}, cp, A, A32, s);
+    else
+      ch_factor_stages<2>(std::make_integer_sequence<int, 4>{}, cp, A, A32, s);
+  }
+
+  // one solve stage: forward (FWD) or backward of colour COL; the backward
+  // stage of the last colour is part of its forward stage
+  template <int DIM, int COL, bool FWD, typename T>
+  static void ch_stage(const ChParams &cp, const T *F, const T *inv_diag, const T *src, T *dst,
+                       hipStream_t s)
+  {
+    if (cp.cnt[COL] == 0)
+      return;
+    T *W = const_cast<T *>(F) + cp.off[8];
+    if constexpr (FWD)
+      k_ch_forward<DIM, COL, T><<<blocks_for(cp.cnt[COL]), 256, 0, s>>>(cp, F, src, W);
+    else if constexpr (COL < (1 << DIM) - 1)
+      k_ch_backward<DIM, COL, T><<<blocks_for(cp.cnt[COL]), 256, 0, s>>>(cp, F, W);
+  }
+
+  template <int DIM, typename T, int... C>
+  static void ch_solves(std::integer_sequence<int, C...>, const ChParams &cp, const T *F,
+                        const T *inv_diag, const T *src, T *dst, hipStream_t s)
+  {
+    // forward in increasing colour order, then backward in decreasing order
+    (ch_stage<DIM, C, true, T>(cp, F, inv_diag, src, dst, s), ...);
+    (ch_stage<DIM, (int)sizeof...(C) - 1 - C, false, T>(cp, F, inv_diag, src, dst, s), ...);
+    const dim3 grid((unsigned)(cp.nn[1] * cp.nn[2]), blocks_for((int64_t)cp.nn[0] * cp.n_comp));
+    k_ch_scatter<DIM, T><<<grid, 256, 0, s>>>(cp, F + cp.off[8], inv_diag, src, dst);
   }
 
   template <typename T>
   void ch_apply(const ChParams &cp, const T *F, const T *inv_diag, const T *src, T *dst,
                 hipStream_t s)
   {
-    const int ncol = 1 << cp.dim;
-    for (int col = 0; col < ncol; ++col)
-      {
-        const int64_t n = colour_count(cp, col);
-        if (n == 0)
-          continue;
-        if (cp.dim == 3)
-          k_ch_forward<3, T><<<blocks_for(n), 256, 0, s>>>(cp, col, F, src, dst);
-        else
-          k_ch_forward<2, T><<<blocks_for(n), 256, 0, s>>>(cp, col, F, src, dst);
-      }
-    for (int col = ncol - 1; col >= 0; --col)
-      {
-        const int64_t n = colour_count(cp, col);
-        if (n == 0)
-          continue;
-        if (cp.dim == 3)
-          k_ch_backward<3, T><<<blocks_for(n), 256, 0, s>>>(cp, col, F, inv_diag, src, dst);
-        else
-          k_ch_backward<2, T><<<blocks_for(n), 256, 0, s>>>(cp, col, F, inv_diag, src, dst);
-      }
+    if (cp.dim == 3)
+      ch_solves<3, T>(std::make_integer_sequence<int, 8>{}, cp, F, inv_diag, src, dst, s);
+    else
+      ch_solves<2, T>(std::make_integer_sequence<int, 4>{}, cp, F, inv_diag, src, dst, s);
   }
   template void ch_apply<double>(const ChParams &, const double *, const double *,
                                  const double *, double *, hipStream_t);

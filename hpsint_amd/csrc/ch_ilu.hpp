@@ -24,25 +24,26 @@ namespace hps
     int        tx, ty, tz, nt[3], nt_lanes, nf;
     double     inv_h[3], jxw;
     PhysParams pp;
-    // storage of the block, colour by colour and entry by entry: entry
-    // (s, a, b) of the node with in-colour index i of colour k at
-    // off[k] + (4 s + 2 a + b) cnt[k] + i (consecutive nodes of a colour
-    // read consecutive addresses); set by ch_layout
-    int64_t    off[9], cnt[8];
+    // storage of the block, colour by colour and stencil entry by stencil
+    // entry: the 2x2 block (a, b) of stencil entry s of the node with
+    // in-colour index i of colour k at off[k] + 4 (s cnt[k] + i) + 2 a + b
+    // (consecutive nodes of a colour read consecutive 4-vectors).  The
+    // colour's nodes form an mx[k] x my[k] x (cnt[k] / (mx[k] my[k])) grid,
+    // x fastest.  After the 8 colours (at off[8]) the solves keep a work
+    // vector of (c, mu) pairs in the same colour-major node order: node i of
+    // colour k at pair noff[k] + i.  Set by ch_layout.
+    int64_t    off[9], cnt[8], noff[9];
+    int        mx[8], my[8];
   };
 
-  // fills cp.off and cp.cnt from the node counts
+  // fills cp.off, cp.cnt, cp.noff, cp.mx and cp.my from the node counts
   void ch_layout(ChParams &cp);
-
-  // per colour: the lower-colour neighbours as stencil indices, ordered by
-  // (colour, index); n_lower of them
-  struct ChOrder
+  // values (of the factor's type) to allocate: the factor and the work vector
+  inline int64_t ch_storage(const ChParams &cp)
   {
-    signed char order[8][26];
-    int         n_lower[8];
-  };
+    return cp.off[8] + 2 * cp.noff[8];
+  }
 
-  ChOrder ch_order(int dim);
   // Cahn-Hilliard block of the tangent (FP64) from the FP64
   // linearisation cache (scalar mobility)
   void ch_assemble(const ChParams &cp, const double *cache, double *A, hipStream_t s);

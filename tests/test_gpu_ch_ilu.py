@@ -59,7 +59,8 @@ def check(mesh, G, u, dt=1e-3, seed=3):
     assert np.array_equal(host(y2), host(y))
 
 
-@pytest.mark.parametrize("cells,G", [((9, 5, 7), 2), ((6, 7, 5), 4), ((13, 9), 2), ((8, 11), 3)])
+@pytest.mark.parametrize("cells,G", [((9, 5, 7), 2), ((6, 7, 5), 4), ((13, 9), 2), ((8, 11), 3),
+                                     ((5, 6, 4), 8), ((4, 5, 3), 12)])
 def test_ch_ilu_random_fields(cells, G):
     dim = len(cells)
     h = (0.5, 0.75, 1.0)[:dim]
