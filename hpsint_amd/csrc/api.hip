@@ -70,6 +70,9 @@ struct hps_operator_s
   hps::KParams         kp;
   hps::FixParams       fp;
   int                  n_tiles, tz;
+  // 3D scalar tangent with z-column lanes (8 x 8 x tz tiles, ntz1 along y)
+  bool                 zcol;
+  int                  ntz1, n_tiles_z;
   int64_t              n_nodes, n_dofs, cache_elems, part_elems;
   hipStream_t          stream;
   double              *cache64;
@@ -283,6 +286,18 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
       fp.n_m    = (int64_t)fp.nb[2] * fp.nn[0] * fp.nn[1];
       fp.n_y    = (int64_t)fp.nb[1] * fp.nn[0] * fp.nn[2];
       fp.n_x    = (int64_t)fp.nb[0] * fp.nn[1] * fp.nn[2];
+
+      // the z-column tangent (3D scalar mobility, up to ZCOL_MAX_G order
+      // parameters, the cache of 8 x 4 tiles); HPSINT_AMD_ZCOL=0 keeps the
+      // one-lane-per-corner kernel
+      op->zcol = prm->dim == 3 && !tens && G <= hps::ZCOL_MAX_G && ops->tx == hps::TX3 &&
+                 ops->ty == hps::TY3 && env_int("HPSINT_AMD_ZCOL", 1) != 0;
+      if (op->zcol)
+        {
+          op->ntz1       = (ncy + 7) / 8;
+          op->n_tiles_z  = kp.nt[0] * op->ntz1 * kp.nt[2];
+          op->part_elems = std::max(op->part_elems, hps::zcol_part_elems(kp, op->ntz1, op->nc));
+        }
       if ((e = hipMalloc(&op->part, op->part_elems * sizeof(double))) != hipSuccess)
         return cleanup_fail(e);
     }
@@ -513,10 +528,17 @@ extern "C" int hps_set_linearization(hps_operator op, const double *u)
 // sums the tile-face partials of the Q1 kernels (the Q2 colour passes
 // complete dst themselves)
 template <typename T>
-static int finish_fixup(hps_operator op, T *dst, int ostride, int ocstride, int ooffset)
+static int finish_fixup(hps_operator op, T *dst, int ostride, int ocstride, int ooffset,
+                        bool zcol_tiles = false)
 {
   if (op->q2)
     return HPS_OK;
+  if (zcol_tiles)
+    {
+      hps::fixup_zcol_d3(op->kp, op->ntz1, op->nc, (const T *)op->part, dst, op->stream);
+      HPS_LAUNCHED();
+      return HPS_OK;
+    }
   hps::FixParams fp = op->fp;
   fp.ostride        = ostride;
   fp.ocstride       = ocstride;
@@ -631,6 +653,9 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
           hps::vmult_cut_d3(g, op->kp, op->n_grains, op->cut_tiles + op->cut_off[g], count,
                             op->cut_masks, src, op->cache64, dst, op->part, op->stream);
       }
+  else if (op->zcol && !op->cut_active)
+    hps::vmult_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, src, op->cache64, dst,
+                       op->part, op->stream);
   else
     (op->cut_active ? op->ops->vmult64_cut : op->ops->vmult64)(op->kp, op->n_tiles, src, dst,
                                                                op->part, op->cache64,
@@ -638,7 +663,8 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
-  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0))
+  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0,
+                                   !op->cut_ready && op->zcol && !op->cut_active))
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));
@@ -679,13 +705,17 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
                             op->cut_masks, src, op->cache32, dst, (float *)op->part,
                             op->stream);
       }
+  else if (op->zcol && !op->cut_active)
+    hps::vmult_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, src, op->cache32, dst,
+                       (float *)op->part, op->stream);
   else
     (op->cut_active ? op->ops->vmult32_cut : op->ops->vmult32)(
       op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
-  if (int r = finish_fixup<float>(op, dst, op->nc, 1, 0))
+  if (int r = finish_fixup<float>(op, dst, op->nc, 1, 0,
+                                  !op->cut_ready && op->zcol && !op->cut_active))
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));

@@ -267,4 +267,20 @@ namespace hps
   bool vmult_cut_d3(int ga, const KParams &kp, int n_grains, const int *tiles, int count,
                     const unsigned *masks, const float *src, const float *cache, float *dst,
                     float *part, hipStream_t s);
+
+  // the 3D scalar-mobility tangent with z-column lanes (fe_q1_zcol.hpp):
+  // 8 x 8 x tz tiles, ntz1 of them along y; reads the cache of the TX3 x
+  // TY3 tiles.  False when n_grains is not built for it.
+  constexpr int ZCOL_MAX_G = 4;
+  bool vmult_zcol_d3(int n_grains, const KParams &kp, int ntz1, int n_tiles, const double *src,
+                     const double *cache, double *dst, double *part, hipStream_t s);
+  bool vmult_zcol_d3(int n_grains, const KParams &kp, int ntz1, int n_tiles, const float *src,
+                     const float *cache, float *dst, float *part, hipStream_t s);
+  // its shared-node fix-up (partials slot-major per boundary class) and the
+  // size of its partial buffer (values)
+  void    fixup_zcol_d3(const KParams &kp, int ntz1, int n_comp, const double *part,
+                        double *dst, hipStream_t s);
+  void    fixup_zcol_d3(const KParams &kp, int ntz1, int n_comp, const float *part, float *dst,
+                        hipStream_t s);
+  int64_t zcol_part_elems(const KParams &kp, int ntz1, int n_comp);
 } // namespace hps

@@ -474,3 +474,26 @@ def test_every_grain_count_is_built():
                 with pytest.raises(_ffi.InvalidNumberOfComponents):
                     SinteringOperatorGeneric(mesh, 13, FreeEnergyParams(),
                                              MobilityParams(tensorial=tens), fe_degree=deg)
+
+
+@pytest.mark.parametrize("zcol", ["0", "1"])
+@pytest.mark.parametrize("cells,G", [((19, 13, 37), 4), ((9, 5, 17), 2), ((3, 7, 2), 3),
+                                     ((8, 12, 16), 1), ((17, 9, 33), 4)])
+def test_vmult_zcol_and_corner_lanes(monkeypatch, zcol, cells, G):
+    """the 3D scalar tangent with z-column lanes (8 x 8 tiles) and with one
+    lane per cell corner give the oracle's vmult, FP64 at 1e-12 and FP32 per
+    field at 1e-5"""
+    monkeypatch.setenv("HPSINT_AMD_ZCOL", zcol)
+    op, p = make_pair(3, cells, G, False, w0=100.0)
+    u = fields(p, 3)
+    src = np.random.default_rng(4).uniform(-1, 1, size=u.size)
+    ref = po.vmult(p, po.fill_qp(p, u), src)
+    op.fill_quadrature_point_values(dev(u))
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, dev(src))
+    assert rel_l2(host(dst), ref) <= TOL64, rel_l2(host(dst), ref)
+    d32 = op.initialize_dof_vector(torch.float32)
+    op.vmult(d32, dev(src, torch.float32))
+    g, r = host(d32).reshape(-1, G + 2), ref.reshape(-1, G + 2)
+    errs = [rel_l2(g[:, c], r[:, c]) for c in range(G + 2)]
+    assert max(errs) <= 1e-5, errs
