@@ -568,6 +568,7 @@ def main():
     old = [u.clone()]
     r = op.initialize_dof_vector()
     rs = lambda: op.evaluate_nonlinear_residual(r, u, 2, old=old)  # noqa: E731
+    settle(rs, 0.4 * args.settle)
     for _ in range(args.warmup):
         rs()
     t_res = timed(rs, args.steps)
@@ -575,6 +576,7 @@ def main():
     src32 = src.float()
     dst32 = op.initialize_dof_vector(torch.float32)
     vm32 = lambda: op.vmult(dst32, src32)  # noqa: E731
+    settle(vm32, 0.4 * args.settle)
     for _ in range(args.warmup):
         vm32()
     t_vm32 = timed(vm32, args.steps)
@@ -630,8 +632,10 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": VMULT_TRAFFIC_128 if (n == 128 and G == 4) else None,
-            "kernel": "q1_apply FP64 tangent (main kernel; its mean duration from HIP events "
-                      "over %d back-to-back vmults)" % args.steps,
+            "kernel": "%s FP64 tangent (main kernel; its mean duration from HIP events "
+                      "over %d back-to-back vmults)"
+                      % ("q1_apply" if os.environ.get("HPSINT_AMD_ZCOL") == "0" else "q1_vmult_zcol",
+                         args.steps),
             "whole_vmult_frac": bytes_vmult / t_vmult / 1e9 / HBM_PEAK_GBS,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on the "
                               "128^3, 4-order-parameter workload, scaled by a copy of known "
