@@ -3,7 +3,9 @@
     python -m hpsint_amd.build          # or hpsint_amd.build.build()
 
 Every translation unit under hpsint_amd/csrc/*.hip is compiled with hipcc
-for gfx950 in parallel, then linked into hpsint_amd/libhpsint_amd.so.
+for gfx950 in parallel, then linked into hpsint_amd/libhpsint_amd.so; the
+C++ adapter (hpsint_amd/adapter/) is compiled against the C ABI into
+hpsint_amd/libhpsint_adapter_check.so.
 """
 import concurrent.futures
 import os
@@ -14,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhpsint_amd.so")
+ADAPTER_LIB = os.path.join(HERE, "libhpsint_adapter_check.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -41,14 +44,31 @@ def build(jobs=None, verbose=True):
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC",
+           "-Wl,-soname,libhpsint_amd.so", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-8000:])
     os.replace(tmp, LIB)
+    build_adapter_check()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", ADAPTER_LIB)
     return LIB
+
+
+def build_adapter_check():
+    """the C++ adapter of adapter/sintering_operator_mi355x.h compiled
+    against include/hpsint_amd.h and linked to the library, with a C entry
+    point per adapter call (adapter/adapter_check.cpp) for the in-process
+    test"""
+    tmp = ADAPTER_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-shared", "-fPIC", "-o", tmp,
+           os.path.join(HERE, "adapter", "adapter_check.cpp"), "-L" + HERE, "-l:libhpsint_amd.so",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("adapter build failed:\n" + r.stderr[-8000:])
+    os.replace(tmp, ADAPTER_LIB)
 
 
 if __name__ == "__main__":

@@ -38,3 +38,16 @@ def test_invalid_arguments_fail_without_gpu():
     with pytest.raises(_ffi.InvalidNumberOfComponents):
         _ffi.check(_ffi.HPS_ERR_INVALID_N_COMP)
     assert L.hps_vmult_f64(None, None, None) == _ffi.HPS_ERR_INVALID_ARGUMENT
+
+
+def test_adapter_library_loads_in_process():
+    """the C++ adapter compiled against the header links the library and
+    exports its check entry points (loaded in this process, no compute)"""
+    path = os.path.join(os.path.dirname(_ffi.LIB_PATH), "libhpsint_adapter_check.so")
+    if not os.path.exists(path):
+        pytest.fail("libhpsint_adapter_check.so not built")
+    _ffi.lib()
+    A = ctypes.CDLL(path)
+    for name in ("hpsad_create", "hpsad_destroy", "hpsad_set_time", "hpsad_fill",
+                 "hpsad_vmult", "hpsad_residual", "hpsad_precond", "hpsad_hook_calls"):
+        assert hasattr(A, name), name
