@@ -71,7 +71,7 @@ struct hps_operator_s
   hps::FixParams       fp;
   int                  n_tiles, tz;
   // 3D scalar tangent with z-column lanes (8 x 8 x tz tiles, ntz1 along y)
-  bool                 zcol;
+  bool                 zcol, zcol_residual;
   int                  ntz1, n_tiles_z;
   int64_t              n_nodes, n_dofs, cache_elems, part_elems;
   hipStream_t          stream;
@@ -292,6 +292,8 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
       // one-lane-per-corner kernel
       op->zcol = prm->dim == 3 && !tens && G <= hps::ZCOL_MAX_G && ops->tx == hps::TX3 &&
                  ops->ty == hps::TY3 && env_int("HPSINT_AMD_ZCOL", 1) != 0;
+      // the residual with z-column lanes as well (HPSINT_AMD_ZCOL_RES=0: not)
+      op->zcol_residual = op->zcol && env_int("HPSINT_AMD_ZCOL_RES", 1) != 0;
       if (op->zcol)
         {
           op->ntz1       = (ncy + 7) / 8;
@@ -571,15 +573,19 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
     HPS_HIP(hipEventRecord(timing_event(op, 0), op->stream));
   // with order 0 there is no history: evaluate without the history term
   const int w = (wtd == 2 && op->kp.order == 0) ? 1 : wtd;
+  const bool zres = op->zcol && !op->cut_active && op->zcol_residual;
   if (op->q2)
     op->q2->residual(q2_params(op), w, src, o[0], o[1], o[2], dst, op->stream);
+  else if (zres)
+    hps::residual_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, w, src, o[0], o[1],
+                          o[2], dst, op->part, op->stream);
   else
     (op->cut_active ? op->ops->residual_cut : op->ops->residual)(
       op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
-  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0))
+  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0, zres))
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));

@@ -1,6 +1,7 @@
-// fe_q1_zcol.hpp -- the 3D scalar-mobility tangent vmult with z-column
-// lanes (the Q1 operator of S/operator_sintering_generic.h:539-577, 104-190
-// with MobilityScalar, S/mobility.h:477-534).
+// fe_q1_zcol.hpp -- the 3D scalar-mobility tangent vmult and nonlinear
+// residual with z-column lanes (the Q1 operator of S/operator_sintering_
+// generic.h:539-577, 104-190 and 877-1006, 235-321 with MobilityScalar,
+// S/mobility.h:441-534).
 //
 // Lane mapping.  One lane per (cell, x/y corner): lane = 4 cell + (bx | by
 // << 1), cell = cx + 8 cy, so a workgroup of 256 lanes holds 8 x 8 cells of
@@ -225,15 +226,89 @@ namespace hps
     oh          = (Uh + Xyh) + xshfl<1>(r * Uh - Xyh);
   }
 
+  // the same butterflies unscaled (value levels sa own + sb partner), for
+  // the residual, whose physics needs the true values; derivatives
+  // unsigned (own minus partner)
+  template <typename T>
+  __device__ __forceinline__ void zcol_interp_plain(T ul, T uh, T &vl, T &vh, T &gxl, T &gxh,
+                                                    T &gyl, T &gyh, T &gzl)
+  {
+    const T sa  = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    const T pl  = xshfl<1>(ul);
+    const T ph  = xshfl<1>(uh);
+    const T Al  = sa * ul + sb * pl;
+    const T Ah  = sa * uh + sb * ph;
+    const T Dl  = ul - pl;
+    const T Dh  = uh - ph;
+    const T pAl = xshfl<2>(Al);
+    const T pAh = xshfl<2>(Ah);
+    const T pDl = xshfl<2>(Dl);
+    const T pDh = xshfl<2>(Dh);
+    const T AAl = sa * Al + sb * pAl;
+    const T AAh = sa * Ah + sb * pAh;
+    const T Ayl = Al - pAl;
+    const T Ayh = Ah - pAh;
+    const T DAl = sa * Dl + sb * pDl;
+    const T DAh = sa * Dh + sb * pDh;
+    vl          = sa * AAl + sb * AAh;
+    vh          = sa * AAh + sb * AAl;
+    gzl         = AAl - AAh;
+    gxl         = sa * DAl + sb * DAh;
+    gxh         = sa * DAh + sb * DAl;
+    gyl         = sa * Ayl + sb * Ayh;
+    gyh         = sa * Ayh + sb * Ayl;
+  }
+
+  // values only (same levels, same rounding as zcol_interp_plain)
+  template <typename T>
+  __device__ __forceinline__ void zcol_interp_values_plain(T ul, T uh, T &vl, T &vh)
+  {
+    const T sa  = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    const T Al  = sa * ul + sb * xshfl<1>(ul);
+    const T Ah  = sa * uh + sb * xshfl<1>(uh);
+    const T AAl = sa * Al + sb * xshfl<2>(Al);
+    const T AAh = sa * Ah + sb * xshfl<2>(Ah);
+    vl          = sa * AAl + sb * AAh;
+    vh          = sa * AAh + sb * AAl;
+  }
+
+  // transpose of zcol_interp_plain; the gradients given as the gradient row
+  // times the signed JxW / h_d of the Gauss point's corner (integrate_vg)
+  template <typename T>
+  __device__ __forceinline__ void zcol_integrate_plain(T vl, T vh, const T *gl, const T *gh,
+                                                       T &ol, T &oh)
+  {
+    const T sa  = Q1Coef<T>::sa, sb = Q1Coef<T>::sb;
+    const T Zl  = (sa * vl + gl[2]) + (sb * vh - gh[2]);
+    const T Zh  = (sa * vh + gh[2]) + (sb * vl - gl[2]);
+    const T Xl  = sa * gl[0] + sb * gh[0];
+    const T Xh  = sa * gh[0] + sb * gl[0];
+    const T Yl  = sa * gl[1] + sb * gh[1];
+    const T Yh  = sa * gh[1] + sb * gl[1];
+    const T Ul  = (sa * Zl + Yl) + xshfl<2>(sb * Zl - Yl);
+    const T Uh  = (sa * Zh + Yh) + xshfl<2>(sb * Zh - Yh);
+    const T Xyl = sa * Xl + sb * xshfl<2>(Xl);
+    const T Xyh = sa * Xh + sb * xshfl<2>(Xh);
+    ol          = (sa * Ul + Xyl) + xshfl<1>(sb * Ul - Xyl);
+    oh          = (sa * Uh + Xyh) + xshfl<1>(sb * Uh - Xyh);
+  }
+
   // ntz1: tiles along y (8 cells each); the workgroups are the tiles
   // (x fastest, then y, then the march direction).  W: waves per SIMD asked
-  // of the register allocation.
-  template <int G, typename T, int W>
+  // of the register allocation.  OP: the tangent (OP_VMULT, reads the
+  // cache) or the residual (OP_RES0/1/2, S/operator_sintering_generic.h:
+  // 877-1006 with MobilityScalar; OP_RES2 reads the old solutions, FP64).
+  template <int G, typename T, int W, int OP = OP_VMULT>
   __global__ void __launch_bounds__(zcol::NT) __attribute__((amdgpu_waves_per_eu(W)))
     q1_vmult_zcol(const KParams kp, const ZSlots zs, int ntz1, const T *__restrict__ src,
-                  const T *__restrict__ cache, T *__restrict__ dst, T *__restrict__ part)
+                  const T *__restrict__ cache, T *__restrict__ dst, T *__restrict__ part,
+                  const T *__restrict__ old0 = nullptr, const T *__restrict__ old1 = nullptr,
+                  const T *__restrict__ old2 = nullptr)
   {
     using CL            = CacheLayout<3, G, false>;
+    constexpr bool VM   = OP == OP_VMULT;
+    constexpr bool HIST = OP == OP_RES2;
+    constexpr int  NPL  = HIST ? 2 : 1; // planes per ring slot: src, history
     constexpr int NC    = G + 2;
     constexpr int NF    = CL::n_fields;
     constexpr int NT    = zcol::NT;
@@ -242,7 +317,7 @@ namespace hps
     constexpr int ITEMS = (PN * NC + NT - 1) / NT;
     constexpr int NTC   = 256; // lanes of a cache tile (8 x 4 cells x 8 corners)
 
-    __shared__ T ring[2][PN * NC];
+    __shared__ T ring[2][NPL][PN * NC];
     __shared__ T slots[4][PN * NC];
 
     const int tid  = threadIdx.x;
@@ -320,18 +395,39 @@ namespace hps
              c;
     };
 
-    T    regs[ITEMS];
+    // src (and, with the history, the combination of the old solutions
+    // with their BDF weights, do_evalute_history, S/operator_sintering_
+    // generic.h:950-960) of node plane k
+    T    regs[NPL][ITEMS];
     auto load_plane = [&](int k) {
 #pragma unroll
       for (int r = 0; r < ITEMS; ++r)
         if (it_ok[r])
-          regs[r] = it_in[r] ? src[it_g0[r] + k * pstride] : T(0);
+          {
+            const int64_t gi = it_g0[r] + k * pstride;
+            regs[0][r]       = it_in[r] ? src[gi] : T(0);
+            if constexpr (HIST)
+              {
+                T h = T(0);
+                if (it_in[r])
+                  {
+                    h = old0[gi] * T(kp.w_hist[0]);
+                    if (kp.order >= 2)
+                      h += old1[gi] * T(kp.w_hist[1]);
+                    if (kp.order >= 3)
+                      h += old2[gi] * T(kp.w_hist[2]);
+                  }
+                regs[1][r] = h;
+              }
+          }
     };
     auto store_plane = [&](int slot) {
 #pragma unroll
       for (int r = 0; r < ITEMS; ++r)
         if (it_ok[r])
-          ring[slot][tid + r * NT] = regs[r];
+#pragma unroll
+          for (int a = 0; a < NPL; ++a)
+            ring[slot][a][tid + r * NT] = regs[a][r];
     };
     // plane k: the sum over the x/y corner slots of the valid cells, to dst
     // or to the node's partial record
@@ -369,17 +465,28 @@ namespace hps
     for (int c = 0; c < NC; ++c)
       carry[c] = T(0);
 
-    // the cache fields of layer k's two Gauss points
-    T    lin_next[2][NF];
+    // the cache fields of layer k's two Gauss points (tangent)
+    T    lin_next[VM ? 2 : 1][VM ? NF : 1];
     auto load_lin = [&](int k) {
-      const T *cl = cache_t + (int64_t)k * (NF * NTC);
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
+      if constexpr (VM)
         {
-          lin_next[0][f] = cl[f * NTC + lane_lo];
-          lin_next[1][f] = cl[f * NTC + lane_hi];
+          const T *cl = cache_t + (int64_t)k * (NF * NTC);
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            {
+              lin_next[0][f] = cl[f * NTC + lane_lo];
+              lin_next[1][f] = cl[f * NTC + lane_hi];
+            }
         }
     };
+    // residual: the gradient rows' uniform scales JxW / h_d^2 (the sign of
+    // a Gauss point's derivative and of its test function cancel) and the
+    // value rows' JxW
+    T k2[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      k2[d] = T(kp.jxw * kp.inv_h[d] * kp.inv_h[d]);
+    const T jxw = T(kp.jxw);
 
     load_plane(0);
     store_plane(0);
@@ -396,62 +503,115 @@ namespace hps
 
     for (int k = 0; k < tm_eff; ++k)
       {
-        T lin[2][NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
+        const T *plo = &ring[k & 1][0][pn_lane * NC];
+        const T *phi = &ring[(k + 1) & 1][0][pn_lane * NC];
+        if constexpr (!VM)
           {
-            lin[0][f] = lin_next[0][f];
-            lin[1][f] = lin_next[1][f];
-          }
-        const T *plo = &ring[k & 1][pn_lane * NC];
-        const T *phi = &ring[(k + 1) & 1][pn_lane * NC];
-        T        val[2][NC], gx[2][NC], gy[2][NC], gz[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          zcol_interp(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c], gy[0][c],
-                      gy[1][c], gz[c]);
-        __builtin_amdgcn_sched_barrier(0);
-
-        // value rows and the c-row mobility terms per Gauss point
-        T vr[2][NC], M[2], s[2];
-#pragma unroll
-        for (int n = 0; n < 2; ++n)
-          {
-            T g[NC][3], gr_unused[NC][3];
+            T val[2][NC], gx[2][NC], gy[2][NC], gz[NC], hist[2][NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c)
+              zcol_interp_plain(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c],
+                                gy[0][c], gy[1][c], gz[c]);
+            if constexpr (HIST)
               {
-                g[c][0] = gx[n][c];
-                g[c][1] = gy[n][c];
-                g[c][2] = n ? -gz[c] : gz[c];
-              }
-            tangent_qp_rows<3, G, false, true, false>(kp.pp, lin[n], val[n], g, vr[n],
-                                                      gr_unused);
-            scalar_mobility_terms<G>(kp.pp, lin[n][CL::c], lin[n] + CL::eta, val[n], M[n], s[n]);
-          }
-        // gradient rows, one component at a time, and the integration
+                const T *hlo = &ring[k & 1][1][pn_lane * NC];
+                const T *hhi = &ring[(k + 1) & 1][1][pn_lane * NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-          {
-            // the gradient a row needs: grad mu for the c row, grad c for
-            // the mu row, its own for an eta row
-            const int src_c = c == 0 ? 1 : (c == 1 ? 0 : c);
-            T         gs[2][3];
+                for (int c = 0; c < NC; ++c)
+                  zcol_interp_values_plain(hlo[c], hhi[c], hist[0][c], hist[1][c]);
+              }
+            __builtin_amdgcn_sched_barrier(0);
+            // value rows and the mobility of the c row per Gauss point
+            T vr[2][NC], M[2];
 #pragma unroll
             for (int n = 0; n < 2; ++n)
               {
-                const T gg[3]  = {gx[n][src_c], gy[n][src_c], n ? -gz[src_c] : gz[src_c]};
-                const T gsj[3] = {fs.gsj[0], fs.gsj[1], n ? gsz : -gsz};
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                  gs[n][d] = c == 0 ? (M[n] * fs.kj[d]) * gg[d] +
-                                        s[n] * (lin[n][CL::grad_mu + d] * gsj[d])
-                                    : (c == 1 ? fs.kmu[d] * gg[d] : fs.keta[d] * gg[d]);
+                residual_values<3, G, false, OP - OP_RES0>(kp.pp, val[n], hist[n], vr[n]);
+                const T one[3] = {T(1), T(1), T(1)};
+                T       m3[3];
+                residual_grad_c_scalar<3, G>(kp.pp, val[n], one, m3);
+                M[n] = m3[0];
               }
-            T ol, oh;
-            zcol_integrate(vr[0][c] * jxw_hat, vr[1][c] * jxw_hat, gs[0], gs[1], ol, oh);
-            slots[xy][pn_lane * NC + c] = ol + carry[c];
-            carry[c]                    = oh;
+            // gradient rows M grad mu, -kappa_c grad c, L kappa_p grad eta_i
+            const T kc = -T(kp.pp.kappa_c), lkp = T(kp.pp.L) * T(kp.pp.kappa_p);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              {
+                const int src_c = c == 0 ? 1 : (c == 1 ? 0 : c);
+                T         gs[2][3];
+#pragma unroll
+                for (int n = 0; n < 2; ++n)
+                  {
+                    const T f     = c == 0 ? M[n] : (c == 1 ? kc : lkp);
+                    const T gg[3] = {gx[n][src_c], gy[n][src_c], n ? -gz[src_c] : gz[src_c]};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                      gs[n][d] = (f * gg[d]) * k2[d];
+                  }
+                T ol, oh;
+                zcol_integrate_plain(vr[0][c] * jxw, vr[1][c] * jxw, gs[0], gs[1], ol, oh);
+                slots[xy][pn_lane * NC + c] = ol + carry[c];
+                carry[c]                    = oh;
+              }
+          }
+        else
+          {
+            T lin[2][NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+              {
+                lin[0][f] = lin_next[0][f];
+                lin[1][f] = lin_next[1][f];
+              }
+            T        val[2][NC], gx[2][NC], gy[2][NC], gz[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              zcol_interp(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c], gy[0][c],
+                          gy[1][c], gz[c]);
+            __builtin_amdgcn_sched_barrier(0);
+
+            // value rows and the c-row mobility terms per Gauss point
+            T vr[2][NC], M[2], s[2];
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+              {
+                T g[NC][3], gr_unused[NC][3];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                  {
+                    g[c][0] = gx[n][c];
+                    g[c][1] = gy[n][c];
+                    g[c][2] = n ? -gz[c] : gz[c];
+                  }
+                tangent_qp_rows<3, G, false, true, false>(kp.pp, lin[n], val[n], g, vr[n],
+                                                          gr_unused);
+                scalar_mobility_terms<G>(kp.pp, lin[n][CL::c], lin[n] + CL::eta, val[n], M[n],
+                                         s[n]);
+              }
+            // gradient rows, one component at a time, and the integration
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              {
+                // the gradient a row needs: grad mu for the c row, grad c for
+                // the mu row, its own for an eta row
+                const int src_c = c == 0 ? 1 : (c == 1 ? 0 : c);
+                T         gs[2][3];
+#pragma unroll
+                for (int n = 0; n < 2; ++n)
+                  {
+                    const T gg[3]  = {gx[n][src_c], gy[n][src_c], n ? -gz[src_c] : gz[src_c]};
+                    const T gsj[3] = {fs.gsj[0], fs.gsj[1], n ? gsz : -gsz};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                      gs[n][d] = c == 0 ? (M[n] * fs.kj[d]) * gg[d] +
+                                            s[n] * (lin[n][CL::grad_mu + d] * gsj[d])
+                                        : (c == 1 ? fs.kmu[d] * gg[d] : fs.keta[d] * gg[d]);
+                  }
+                T ol, oh;
+                zcol_integrate(vr[0][c] * jxw_hat, vr[1][c] * jxw_hat, gs[0], gs[1], ol, oh);
+                slots[xy][pn_lane * NC + c] = ol + carry[c];
+                carry[c]                    = oh;
+              }
           }
         __syncthreads();
         if (k + 2 <= tm_eff)

@@ -31,6 +31,44 @@ namespace hps
       q1_vmult_zcol<G, T, 4><<<n_tiles, zcol::NT, 0, s>>>(kp, zs, ntz1, src, cache, dst, part);
   }
 
+  // the residual (FP64) with z-column lanes, wtd 0, 1 or 2
+  template <int G>
+  static void zcol_residual(int wtd, const KParams &kp, int ntz1, int n_tiles, const double *src,
+                            const double *o0, const double *o1, const double *o2, double *dst,
+                            double *part, hipStream_t s)
+  {
+    const ZSlots zs = zcol_slots(kp, ntz1, G + 2);
+    if (wtd == 0)
+      q1_vmult_zcol<G, double, 2, OP_RES0>
+        <<<n_tiles, zcol::NT, 0, s>>>(kp, zs, ntz1, src, nullptr, dst, part, o0, o1, o2);
+    else if (wtd == 1)
+      q1_vmult_zcol<G, double, 2, OP_RES1>
+        <<<n_tiles, zcol::NT, 0, s>>>(kp, zs, ntz1, src, nullptr, dst, part, o0, o1, o2);
+    else
+      q1_vmult_zcol<G, double, 2, OP_RES2>
+        <<<n_tiles, zcol::NT, 0, s>>>(kp, zs, ntz1, src, nullptr, dst, part, o0, o1, o2);
+  }
+
+  bool residual_zcol_d3(int n_grains, const KParams &kp, int ntz1, int n_tiles, int wtd,
+                        const double *src, const double *o0, const double *o1, const double *o2,
+                        double *dst, double *part, hipStream_t s)
+  {
+    switch (n_grains)
+      {
+#define HPS_CASE(G)                                                                  \
+  case G:                                                                            \
+    zcol_residual<G>(wtd, kp, ntz1, n_tiles, src, o0, o1, o2, dst, part, s);         \
+    return true;
+        HPS_CASE(1)
+        HPS_CASE(2)
+        HPS_CASE(3)
+        HPS_CASE(4)
+#undef HPS_CASE
+        default:
+          return false;
+      }
+  }
+
   template <typename T>
   static bool zcol_launch(int n_grains, const KParams &kp, int ntz1, int n_tiles, const T *src,
                           const T *cache, T *dst, T *part, hipStream_t s)

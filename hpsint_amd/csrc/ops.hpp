@@ -283,4 +283,9 @@ namespace hps
   void    fixup_zcol_d3(const KParams &kp, int ntz1, int n_comp, const float *part, float *dst,
                         hipStream_t s);
   int64_t zcol_part_elems(const KParams &kp, int ntz1, int n_comp);
+  // the residual (FP64, wtd 0, 1, 2) with z-column lanes; its fix-up is
+  // fixup_zcol_d3
+  bool residual_zcol_d3(int n_grains, const KParams &kp, int ntz1, int n_tiles, int wtd,
+                        const double *src, const double *o0, const double *o1, const double *o2,
+                        double *dst, double *part, hipStream_t s);
 } // namespace hps

@@ -497,3 +497,25 @@ def test_vmult_zcol_and_corner_lanes(monkeypatch, zcol, cells, G):
     g, r = host(d32).reshape(-1, G + 2), ref.reshape(-1, G + 2)
     errs = [rel_l2(g[:, c], r[:, c]) for c in range(G + 2)]
     assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("zcol", ["0", "1"])
+@pytest.mark.parametrize("cells,G", [((19, 13, 37), 4), ((9, 5, 17), 2), ((3, 7, 2), 3),
+                                     ((8, 12, 16), 1)])
+def test_residual_zcol_and_corner_lanes(monkeypatch, zcol, cells, G):
+    """the 3D scalar residual with z-column lanes and with one lane per cell
+    corner give the oracle's residual at 1e-12 for every wtd, BDF2 history
+    with unequal steps"""
+    monkeypatch.setenv("HPSINT_AMD_ZCOL_RES", zcol)
+    dt0, dt1 = 1e-2, 2e-2
+    w1 = -(dt0 + dt1) / (dt0 * dt1)
+    w2 = dt0 / (dt1 * (dt0 + dt1))
+    w0 = -(w1 + w2)
+    op, p = make_pair(3, cells, G, False, w0=w0, w_hist=(w1, w2))
+    u = fields(p, 5)
+    olds = [fields(p, 6), fields(p, 7)]
+    for wtd in (0, 1, 2):
+        ref = po.residual(p, wtd, u, olds)
+        dst = op.initialize_dof_vector()
+        op.evaluate_nonlinear_residual(dst, dev(u), wtd, old=[dev(o) for o in olds])
+        assert rel_l2(host(dst), ref) <= TOL64, (wtd, rel_l2(host(dst), ref))
