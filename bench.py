@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # factors in profiles/vmult_q1_128cubed_hbm_traffic.md (z-column kernel).
 # Valid for that workload only.
 VMULT_TRAFFIC_128 = 1.3687e9
-# FP64 flop per launch of the residual main kernel on the same workload
-# (SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes, FMA = 2;
+# FP64 flop per launch of the residual main kernel (z-column lanes) on the
+# same workload (SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes, FMA = 2;
 # profiles/q1_kernels_128cubed_fp64_counters.md); FP64 vector peak
-RESIDUAL_FLOP_128 = 8.25e9
+RESIDUAL_FLOP_128 = 7.858e9
 FP64_PEAK_TFLOPS = 78.6
 
 

@@ -465,7 +465,11 @@ namespace hps
     for (int c = 0; c < NC; ++c)
       carry[c] = T(0);
 
-    // the cache fields of layer k's two Gauss points (tangent)
+    // the cache fields of layer k's two Gauss points (tangent).  FP64: the
+    // next layer's fields are requested during the current layer (PF); FP32:
+    // at the top of their layer, which leaves the registers of the prefetch
+    // free (3 % faster on the 128^3 workload; FP64 is 1 % faster with it)
+    constexpr bool PF = VM && sizeof(T) == 8;
     T    lin_next[VM ? 2 : 1][VM ? NF : 1];
     auto load_lin = [&](int k) {
       if constexpr (VM)
@@ -499,10 +503,13 @@ namespace hps
     // interpolation
     if (2 <= tm_eff)
       load_plane(2);
-    load_lin(0);
+    if (PF)
+      load_lin(0);
 
     for (int k = 0; k < tm_eff; ++k)
       {
+        if (!PF)
+          load_lin(k);
         const T *plo = &ring[k & 1][0][pn_lane * NC];
         const T *phi = &ring[(k + 1) & 1][0][pn_lane * NC];
         if constexpr (!VM)
@@ -618,7 +625,7 @@ namespace hps
           store_plane(k & 1);
         if (k + 3 <= tm_eff)
           load_plane(k + 3);
-        if (k + 1 < tm_eff)
+        if (PF && k + 1 < tm_eff)
           load_lin(k + 1);
         node_sums(k);
         __syncthreads();
