@@ -70,11 +70,13 @@ namespace hps
       }
     };
 
+    // the row of node (x, y, z) of colour col (uniform across the lanes
+    // where it is called, so the parameter arrays are read by scalar loads)
     template <typename T>
-    __device__ __forceinline__ RowRef<T> row_of(const ChParams &cp, T *F, int x, int y, int z)
+    __device__ __forceinline__ RowRef<T> row_of(const ChParams &cp, T *F, int col, int x, int y,
+                                                int z)
     {
-      const int     col = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
-      const int64_t ic  = (x >> 1) + (int64_t)cp.mx[col] * ((y >> 1) + (int64_t)cp.my[col] * (z >> 1));
+      const int64_t ic = (x >> 1) + (int64_t)cp.mx[col] * ((y >> 1) + (int64_t)cp.my[col] * (z >> 1));
       return RowRef<T>{F + cp.off[col] + 4 * ic, 4 * cp.cnt[col]};
     }
   } // namespace
@@ -145,7 +147,9 @@ namespace hps
       {
         const int     kx = cx + (k & 1), ky = cy + ((k >> 1) & 1);
         const int     kz = DIM == 3 ? cz + ((k >> 2) & 1) : 0;
-        const RowRef<double> ra = row_of(cp, A, kx, ky, kz);
+        // the colour of corner k: the cell parity flipped by the corner's
+        // bits (uniform)
+        const RowRef<double> ra = row_of(cp, A, par ^ k, kx, ky, kz);
 #pragma unroll
         for (int m = 0; m < NV; ++m)
           {
@@ -482,10 +486,10 @@ namespace hps
     W2[cp.noff[COL] + ic] = V2{xc, xm};
   }
 
-  // dst in node order, one thread per entry: (c, mu) from the work vector
-  // (the node's colour and place in its colour's grid), the Allen-Cahn
-  // components inv_diag * src.  Workgroup (row, chunk): blockIdx.x is the
-  // node row (y, z), blockIdx.y the chunk of 256 entries of the row, so the
+  // dst in node order, one workgroup per node row (y, z), its threads
+  // striding over the row's entries: (c, mu) from the work vector (the
+  // node's colour and place in its colour's grid), the Allen-Cahn components
+  // inv_diag * src.  Each thread has several entries in flight, and the
   // index arithmetic stays in 32 bits.
   template <int DIM, typename T>
   __global__ void __launch_bounds__(256)
@@ -494,23 +498,31 @@ namespace hps
   {
     const int nc  = cp.n_comp;
     const int len = cp.nn[0] * nc;
-    const int t   = (int)blockIdx.y * 256 + (int)threadIdx.x;
-    if (t >= len)
-      return;
-    const int     row = (int)blockIdx.x;
-    const int64_t i   = (int64_t)row * len + t;
-    const int     x   = t / nc;
-    const int     c   = t - x * nc;
-    if (c >= 2)
-      {
-        dst[i] = inv_diag[i] * src[i];
-        return;
-      }
+    const int row = (int)blockIdx.x;
+    // the row's two colours (x even, x odd): uniform values, selected per
+    // lane (a per-lane index into the parameter arrays would load them from
+    // memory lane by lane)
     const int     y   = row % cp.nn[1];
     const int     z   = row / cp.nn[1];
-    const int     col = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
-    const int64_t ic  = (x >> 1) + (int64_t)cp.mx[col] * ((y >> 1) + (int64_t)cp.my[col] * (z >> 1));
-    dst[i]            = W[2 * (cp.noff[col] + ic) + c];
+    const int     c0  = ((y & 1) << 1) | ((z & 1) << 2);
+    const int64_t yz0 = (y >> 1) + (int64_t)cp.my[c0] * (z >> 1);
+    const int64_t yz1 = (y >> 1) + (int64_t)cp.my[c0 | 1] * (z >> 1);
+    const int64_t b0  = cp.noff[c0] + (int64_t)cp.mx[c0] * yz0;
+    const int64_t b1  = cp.noff[c0 | 1] + (int64_t)cp.mx[c0 | 1] * yz1;
+    const int64_t r0  = (int64_t)row * len;
+#pragma unroll 4
+    for (int t = (int)threadIdx.x; t < len; t += 256)
+      {
+        const int64_t i = r0 + t;
+        const int     x = t / nc;
+        const int     c = t - x * nc;
+        T             v;
+        if (c >= 2)
+          v = inv_diag[i] * src[i];
+        else
+          v = W[2 * (((x & 1) ? b1 : b0) + (x >> 1)) + c];
+        dst[i] = v;
+      }
   }
 
   // ---------------------------------------------------------------- launchers
@@ -600,8 +612,8 @@ namespace hps
     // forward in increasing colour order, then backward in decreasing order
     (ch_stage<DIM, C, true, T>(cp, F, inv_diag, src, dst, s), ...);
     (ch_stage<DIM, (int)sizeof...(C) - 1 - C, false, T>(cp, F, inv_diag, src, dst, s), ...);
-    const dim3 grid((unsigned)(cp.nn[1] * cp.nn[2]), blocks_for((int64_t)cp.nn[0] * cp.n_comp));
-    k_ch_scatter<DIM, T><<<grid, 256, 0, s>>>(cp, F + cp.off[8], inv_diag, src, dst);
+    k_ch_scatter<DIM, T>
+      <<<(unsigned)(cp.nn[1] * cp.nn[2]), 256, 0, s>>>(cp, F + cp.off[8], inv_diag, src, dst);
   }
 
   template <typename T>
