@@ -48,24 +48,33 @@ namespace hps
       partial[blockIdx.x] = s;
   }
 
-  // sums nb partial rows of k values each: out[j] = sum_b partial[b*k + j];
-  // wave w handles j = w, w + 4, ... (fixed order: deterministic)
+  // sums the RED_BLOCKS partial rows of k values each: out[j] = sum_b
+  // partial[b*k + j].  Block j owns column j; thread t loads rows t, t + RB,
+  // ... all at once and adds them in row order, then the wave trees and the
+  // waves in wave order (fixed order: deterministic).
+  static_assert(RED_BLOCKS % RB == 0, "finish rows per thread");
   __global__ void __launch_bounds__(RB)
-    k_finish(int nb, int k, const double *__restrict__ partial, double *__restrict__ out,
-             int do_sqrt)
+    k_finish(int k, const double *__restrict__ partial, double *__restrict__ out, int do_sqrt)
   {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int j = w; j < k; j += RB / 64)
-      {
-        double s = 0.0;
-        for (int b = lane; b < nb; b += 64)
-          s += partial[(int64_t)b * k + j];
+    constexpr int     PER = RED_BLOCKS / RB;
+    __shared__ double sh[RB / 64];
+    const int         j = blockIdx.x;
+    double            v[PER];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1)
-          s += __shfl_down(s, o, 64);
-        if (lane == 0)
-          out[j] = do_sqrt ? sqrt(s) : s;
-      }
+    for (int i = 0; i < PER; ++i)
+      v[i] = partial[(int64_t)(threadIdx.x + i * RB) * k + j];
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      s += v[i];
+    s = block_reduce(s, sh);
+    if (threadIdx.x == 0)
+      out[j] = do_sqrt ? sqrt(s) : s;
+  }
+
+  static void finish(hipStream_t s, int k, const double *partial, double *out, int do_sqrt)
+  {
+    k_finish<<<k, RB, 0, s>>>(k, partial, out, do_sqrt);
   }
 
   // K values reduced over the block in one pass: per-wave shuffle trees,
@@ -535,20 +544,20 @@ namespace hps
                 double *partial, double *out)
   {
     k_dot<<<RED_BLOCKS, RB, 0, s>>>(n, x, y, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 0);
+    finish(s, 1, partial, out, 0);
   }
 
   void blas_norm2(hipStream_t s, int64_t n, const double *x, double *partial, double *out)
   {
     k_dot<<<RED_BLOCKS, RB, 0, s>>>(n, x, x, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+    finish(s, 1, partial, out, 1);
   }
 
   void blas_block_norms(hipStream_t s, int64_t n_nodes, int nc, const double *x,
                         double *partial, double *out)
   {
     k_block_sq<<<RED_BLOCKS, RB, 0, s>>>(n_nodes, nc, x, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, nc, partial, out, 1);
+    finish(s, nc, partial, out, 1);
   }
 
 #define HPS_K_SWITCH(k, CALL)                                                  \
@@ -582,7 +591,7 @@ namespace hps
         HPS_K_SWITCH(kk, HPS_CALL)
 #undef HPS_CALL
       }
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, k, partial, out, 0);
+    finish(s, k, partial, out, 0);
   }
 
   template <typename T>
@@ -592,7 +601,7 @@ namespace hps
 #define HPS_CALL(K) k_maxpy_norm<K, T><<<RED_BLOCKS, RB, 0, s>>>(n, V, h, w, partial)
     HPS_K_SWITCH(k, HPS_CALL)
 #undef HPS_CALL
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+    finish(s, 1, partial, out, 1);
   }
 
   void blas_multi_dot(hipStream_t s, int64_t n, int k, const double *V, const double *w,
@@ -796,7 +805,7 @@ namespace hps
   void blas_norm2_f32(hipStream_t s, int64_t n, const float *x, double *partial, double *out)
   {
     k_norm_f32<<<RED_BLOCKS, RB, 0, s>>>(n, x, partial);
-    k_finish<<<1, RB, 0, s>>>(RED_BLOCKS, 1, partial, out, 1);
+    finish(s, 1, partial, out, 1);
   }
 
   void blas_scale_f32(hipStream_t s, int64_t n, double a, float *x)
