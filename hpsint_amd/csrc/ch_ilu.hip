@@ -347,11 +347,24 @@ namespace hps
       }
   }
 
+  // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs
+  // on XCD b % 8).  The logical block returned here gives each XCD one
+  // contiguous range of the colour grid, so that the neighbour rows a stage
+  // gathers from the other colours are shared in that XCD's L2.  The map is
+  // a bijection of [0, gridDim.x): XCD x < r takes q + 1 blocks, the others
+  // q, with nb = 8 q + r.
+  __device__ __forceinline__ int xcd_block()
+  {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+  }
+
   template <int DIM, int COL>
   __global__ void __launch_bounds__(256)
     k_ch_factor(const ChParams cp, double *__restrict__ A, float *__restrict__ A32)
   {
-    const int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ic = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (ic >= cp.cnt[COL])
       return;
     const ColourNode<DIM, COL> n(cp, ic);
@@ -444,7 +457,7 @@ namespace hps
     using V2            = typename Vec<T>::V2;
     using V4            = typename Vec<T>::V4;
     constexpr bool LAST = COL == (1 << DIM) - 1;
-    const int64_t  ic   = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t  ic   = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (ic >= cp.cnt[COL])
       return;
     const ColourNode<DIM, COL> n(cp, ic);
@@ -471,7 +484,7 @@ namespace hps
   {
     using V2         = typename Vec<T>::V2;
     using V4         = typename Vec<T>::V4;
-    const int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ic = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (ic >= cp.cnt[COL])
       return;
     const ColourNode<DIM, COL> n(cp, ic);
