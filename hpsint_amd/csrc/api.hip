@@ -45,9 +45,6 @@ namespace
     }                                                                              \
   while (0)
 
-  // tile geometry: cells along y per workgroup (3D) and cells per tile along
-  // the march direction; HPSINT_AMD_TILE_Y / HPSINT_AMD_TILE_M override the
-  // defaults for measurements
   // pinned host buffer of the reductions (and of the per-block loss partials)
   constexpr int RED_HOST_DOUBLES = 4096;
 
@@ -164,10 +161,17 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   const bool tens = prm->mobility == HPS_MOBILITY_TENSORIAL;
   if (G < 1)
     return fail(HPS_ERR_INVALID_N_COMP, "n_components must be >= 3");
+  // tile geometry: cells along y per workgroup (3D) and cells per tile along
+  // the march direction; HPSINT_AMD_TILE_Y / HPSINT_AMD_TILE_M override the
+  // defaults for measurements
   const int tile_y = env_int("HPSINT_AMD_TILE_Y", 4);
   const int TZ     = env_int("HPSINT_AMD_TILE_M", 16);
   if (TZ < 1 || TZ > 4096)
     return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_TILE_M out of range");
+  // Q2 3D tangent vmult: cell layers per tile march (0: colour passes)
+  const int q2_tile = env_int("HPSINT_AMD_Q2_TILE", 2);
+  if (q2_tile < 0 || q2_tile > 4096)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_Q2_TILE out of range");
   const bool          q2  = prm->fe_degree == 2;
   const hps::OpTable *ops = nullptr;
   const hps::Q2Table *q2t = nullptr;
@@ -248,6 +252,7 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
       qp.nc[1]          = prm->n_cells[1];
       qp.nc[2]          = prm->dim == 3 ? prm->n_cells[2] : 1;
       qp.h_prod         = 1.0;
+      qp.tile_zc        = prm->dim == 3 ? q2_tile : 0;
       for (int d = 0; d < 3; ++d)
         {
           qp.nn[d]    = d < prm->dim ? 2 * qp.nc[d] + 1 : 1;

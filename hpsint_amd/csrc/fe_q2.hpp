@@ -57,6 +57,16 @@ namespace hps
     double     h_prod;    // cell volume
     double     w_hist[3]; // BDF weights[1..order]
     PhysParams pp;
+    int        tile_zc; // 3D vmult: cell layers per tile march (0: colour passes)
+  };
+
+  // one colour of the 3D tile march (q2_vmult_tile): tiles of 4 x NY x zc
+  // cells, colour = parity of the tile index along x, y, z
+  struct Q2TileParams
+  {
+    int     tpar[3]; // parity of this colour's tiles
+    int     ntc[3];  // tiles of this colour along x, y, z
+    int64_t wgb[8];  // first cache workgroup of each cell colour
   };
 
   template <typename T>
@@ -183,6 +193,24 @@ namespace hps
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         b0[d] = s < CPW ? l - t[d] * str(d) : 0;
+    }
+    // lane `lane` of a wave whose slot (lane / NP) holds the cell c, which
+    // exists in the mesh if `exists`
+    __device__ Q2Lane(int lane, const int *c, bool exists)
+    {
+      l     = lane;
+      s     = l / NP;
+      p     = l - s * NP;
+      t[0]  = p % 3;
+      t[1]  = (p / 3) % 3;
+      t[2]  = DIM == 3 ? p / 9 : 0;
+      valid = s < CPW && exists;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        {
+          cc[d] = c[d];
+          b0[d] = s < CPW ? l - t[d] * str(d) : 0;
+        }
     }
     __device__ int base(int d) const
     {
@@ -430,6 +458,21 @@ namespace hps
         }
   }
 
+  // barrier of the vmult body: the whole workgroup, or (WAVE) only the
+  // wave, when every wave of the workgroup works on LDS rows of its own
+  template <bool WAVE>
+  __device__ __forceinline__ void q2_sync()
+  {
+    if constexpr (WAVE)
+      {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    else
+      __syncthreads();
+  }
+
   // tangent vmult of the workgroup's cells.  U holds the gathered increment
   // (NC rows of 64) on entry and the integrated rows on exit; A (NC rows) is
   // the second work buffer; VR (NC rows) takes the value rows.  Every
@@ -439,9 +482,10 @@ namespace hps
   // tangent; the gradient of eta_i enters its own row only (L kappa_p
   // grad eta_i) and is formed from the Gauss-point values before the
   // integration.
-  template <int DIM, int G, bool TENS, typename T>
+  // cl: this lane's entry of the first cached field (fields LP entries apart)
+  template <int DIM, int G, bool TENS, typename T, bool WAVE = false>
   __device__ __forceinline__ void q2_vmult_body(const Q2Params &kp, const Q2Lane<DIM> &L, T *U,
-                                                T *VR, T *A, const T *__restrict__ cache,
+                                                T *VR, T *A, const T *__restrict__ cl,
                                                 const T *inv_h)
   {
     using CL         = CacheLayout<DIM, G, TENS>;
@@ -461,13 +505,13 @@ namespace hps
 #pragma unroll
       for (int c = 0; c < NC; ++c)
         A[c * 64 + l] = q2_pass(U + c * 64, L.base(0), 1, srow[0]);
-      __syncthreads();
+      q2_sync<WAVE>();
       if constexpr (DIM == 3)
         {
 #pragma unroll
           for (int c = 0; c < NC; ++c)
             U[c * 64 + l] = q2_pass(A + c * 64, L.base(1), 3, srow[1]);
-          __syncthreads();
+          q2_sync<WAVE>();
 #pragma unroll
           for (int c = 0; c < NC; ++c)
             v[c] = q2_pass(U + c * 64, L.base(2), 9, srow[2]);
@@ -484,7 +528,7 @@ namespace hps
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       QV[c * 64 + l] = v[c];
-    __syncthreads();
+    q2_sync<WAVE>();
 
     T drow[DIM][3];
 #pragma unroll
@@ -505,7 +549,6 @@ namespace hps
       jxw *= q2_w<T>(L.t[d]);
     T             gq[NC][DIM];
     constexpr int LP = LN::LP; // cache: [workgroup][field][LP lanes]
-    const T      *cl = cache + ((kp.wg_base + blockIdx.x) * NF) * LP + l;
     {
       T vr[NC], gc0[DIM];
       if constexpr (TENS && G > 1)
@@ -555,7 +598,7 @@ namespace hps
               q2_pass(QV + c * 64, L.base(d), LN::str(d), drow[d]) * (lkp * inv_h[d] * sd);
         }
     }
-    __syncthreads();
+    q2_sync<WAVE>();
 
     // 4) integration: collocation-derivative transposes of the gradient rows
     //    added to the value rows, then the transposed interpolation passes
@@ -573,39 +616,39 @@ namespace hps
         WS[c * 64 + l] = gq[c][0];
         QV[c * 64 + l] = gq[c][1];
       }
-    __syncthreads();
+    q2_sync<WAVE>();
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       y[c] = VR[c * 64 + l] + q2_pass(WS + c * 64, L.base(0), 1, dcol[0]) +
              q2_pass(QV + c * 64, L.base(1), 3, dcol[1]);
-    __syncthreads();
+    q2_sync<WAVE>();
     if constexpr (DIM == 3)
       {
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           WS[c * 64 + l] = gq[c][2];
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           y[c] += q2_pass(WS + c * 64, L.base(2), 9, dcol[2]);
-        __syncthreads();
+        q2_sync<WAVE>();
         // z, y, x: QV -> WS -> QV -> U (= WS)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           QV[c * 64 + l] = y[c];
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           WS[c * 64 + l] = q2_pass(QV + c * 64, L.base(2), 9, scol[2]);
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           QV[c * 64 + l] = q2_pass(WS + c * 64, L.base(1), 3, scol[1]);
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           y[c] = q2_pass(QV + c * 64, L.base(0), 1, scol[0]);
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           U[c * 64 + l] = y[c];
@@ -616,15 +659,15 @@ namespace hps
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           QV[c * 64 + l] = y[c];
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           WS[c * 64 + l] = q2_pass(QV + c * 64, L.base(1), 3, scol[1]);
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           y[c] = q2_pass(WS + c * 64, L.base(0), 1, scol[0]);
-        __syncthreads();
+        q2_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           U[c * 64 + l] = y[c];
@@ -842,11 +885,167 @@ namespace hps
 
     const T inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
     if constexpr (OP == OP_VMULT)
-      q2_vmult_body<DIM, G, TENS>(kp, L, U, H, A, cache, inv_h);
+      q2_vmult_body<DIM, G, TENS>(kp, L, U, H, A,
+                                  cache + ((kp.wg_base + blockIdx.x) * NF) * Q2Lane<DIM>::LP +
+                                    L.l,
+                                  inv_h);
     else
       q2_residual_body<DIM, G, TENS, OP>(kp, L, U, H, A, inv_h);
     __syncthreads();
     q2_scatter<DIM, NC, T>(kp, L, U, dst, NC, 1, 0);
+  }
+
+  // ------------------------------------------ 3D tangent vmult, tile march
+  // A workgroup owns a tile of 4 x NY cells in (x, y) and marches through up
+  // to tile_zc cell layers along z.  Wave w = xp + 2 r holds the cells
+  // x = 4 tx + xp + {0, 2} of row y = NY ty + r: two cells of one colour, a
+  // cell pair of the linearisation cache, so a wave reads its cache entries
+  // as in the colour passes.  Per layer the waves run the cell bodies on LDS
+  // rows of their own (wave-level barriers); then each node of the tile's
+  // three node planes is summed by one owner lane, the lane of the lowest
+  // cell touching it, from the rows of the up to four cells touching it in
+  // (x, y), in a fixed order, and in the bottom plane the top plane of the
+  // previous layer, carried in LDS.  The two lower planes are written, the
+  // top plane is carried (and written after the last layer).  Tiles of one
+  // colour (parity of the tile index along x, y, z) share no node; a node
+  // on a tile boundary is stored by the first colour that reaches it and
+  // added by the later ones, as in the colour passes.  src and the cache of
+  // the tile are read once per layer; every node of a tile is written once.
+  // WPE: waves per SIMD the register allocation targets
+  template <int G, bool TENS, typename T, int NY, int WPE>
+  __global__ void __launch_bounds__(128 * NY) __attribute__((amdgpu_waves_per_eu(WPE)))
+    q2_vmult_tile(const Q2Params kp, const Q2TileParams tp, const T *__restrict__ src,
+                  const T *__restrict__ cache, T *__restrict__ dst)
+  {
+    using CL          = CacheLayout<3, G, TENS>;
+    using LN          = Q2Lane<3>;
+    constexpr int NC  = G + 2;
+    constexpr int NF  = CL::n_fields;
+    constexpr int NW  = 2 * NY;
+    constexpr int PXY = 9 * (2 * NY + 1); // nodes of a tile plane
+    __shared__ T U[NW][NC * 64];
+    __shared__ T H[NW][NC * 64];
+    __shared__ T A[NW][NC * 64];
+    __shared__ T carry[NC * PXY];
+
+    const int b  = (int)blockIdx.x;
+    const int q  = b / tp.ntc[0];
+    const int tx = 2 * (b - q * tp.ntc[0]) + tp.tpar[0];
+    const int ty = 2 * (q % tp.ntc[1]) + tp.tpar[1];
+    const int tz = 2 * (q / tp.ntc[1]) + tp.tpar[2];
+    // wave index as a uniform value: the cell colour and the cache block
+    // offsets below stay scalar
+    const int w   = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int l   = (int)threadIdx.x & 63;
+    const int xp  = w & 1, cyl = w >> 1;
+    const int sl  = l < 2 * LN::NP ? l / LN::NP : 0;
+    const int cxl = xp + 2 * sl; // cell of the lane in the tile
+    const int cx = 4 * tx + cxl, cy = NY * ty + cyl;
+    const int z0 = kp.tile_zc * tz;
+    const int nl = kp.nc[2] - z0 < kp.tile_zc ? kp.nc[2] - z0 : kp.tile_zc;
+    const bool in_xy = l < 2 * LN::NP && cx < kp.nc[0] && cy < kp.nc[1];
+    // cells of the wave's colour along x and y (cache pair blocks)
+    const int py  = cy & 1;
+    const int ncx = (kp.nc[0] - xp + 1) >> 1, ncy = (kp.nc[1] - py + 1) >> 1;
+    const T   inv_h[3] = {T(kp.inv_h[0]), T(kp.inv_h[1]), T(kp.inv_h[2])};
+    // owner of the lane's node in the tile and the cells adding to it
+    const int  wx = w ^ 1, sx = (cxl + 1) >> 1; // cell (cxl + 1, cyl)
+
+    for (int k = 0; k < nl; ++k)
+      {
+        // the lane index passes through an empty asm statement each layer:
+        // everything derived from it (node and point indices, LDS offsets,
+        // coefficient rows) is recomputed per layer instead of being held in
+        // registers across the march
+        int lk = l;
+        asm volatile("" : "+v"(lk));
+        Q2Params kq = kp;
+        asm volatile("" : "+s"(kq.pp.A), "+s"(kq.pp.B), "+s"(kq.pp.kappa_c), "+s"(kq.pp.kappa_p));
+        asm volatile("" : "+s"(kq.pp.Mvol), "+s"(kq.pp.Mvap), "+s"(kq.pp.Msurf), "+s"(kq.pp.Mgb));
+        asm volatile("" : "+s"(kq.pp.L), "+s"(kq.pp.w0), "+s"(kq.pp.w_alg), "+s"(kq.h_prod));
+        asm volatile("" : "+s"(kq.inv_h[0]), "+s"(kq.inv_h[1]), "+s"(kq.inv_h[2]));
+        const T inv_hk[3] = {T(kq.inv_h[0]), T(kq.inv_h[1]), T(kq.inv_h[2])};
+        const int cz    = z0 + k;
+        const int cc[3] = {cx, cy, cz};
+        const LN  L(lk, cc, in_xy);
+        {
+          T x[NC];
+          if (L.valid)
+            q2_load_row<NC>(src + LN::node(kp, cc, L.p) * NC, x);
+          else
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              x[c] = T(0);
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            U[w][c * 64 + l] = x[c];
+        }
+        q2_sync<true>();
+        const int     col = xp + 2 * py + 4 * (cz & 1);
+        const int64_t ci  = ((int64_t)(cz >> 1) * ncy + (cy >> 1)) * ncx + (cx >> 1);
+        const T      *cl  = cache + ((tp.wgb[col] + (ci >> 1)) * NF) * LN::LP +
+                      (int)(ci & 1) * LN::NP + L.p;
+        q2_vmult_body<3, G, TENS, T, true>(kq, L, U[w], H[w], A[w], cl, inv_hk);
+        __syncthreads();
+
+        const int  t0 = L.t[0], t1 = L.t[1], t2 = L.t[2];
+        const int  pn = (2 * cxl + t0) + 9 * (2 * cyl + t1);
+        const bool own = L.valid && (t0 != 0 || cxl == 0) && (t1 != 0 || cyl == 0);
+        T          v[NC];
+        bool       keep = false;
+        if (own)
+          {
+            const bool xr = t0 == 2 && cxl < 3 && cx + 1 < kp.nc[0];
+            const bool yu = t1 == 2 && cyl < NY - 1 && cy + 1 < kp.nc[1];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              v[c] = U[w][c * 64 + l];
+            if (xr)
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                v[c] += U[wx][c * 64 + sx * LN::NP + L.p - 2];
+            if (yu)
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                v[c] += U[w + 2][c * 64 + l - 6];
+            if (xr && yu)
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                v[c] += U[wx + 2][c * 64 + sx * LN::NP + L.p - 8];
+            if (t2 == 0 && k > 0)
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                v[c] += carry[c * PXY + pn];
+            if (t2 < 2 || k == nl - 1)
+              {
+                const int  gx = 2 * cx + t0, gy = 2 * cy + t1, gz = 2 * cz + t2;
+                const bool shx = (gx == 8 * tx && tx > 0) ||
+                                 (gx == 8 * tx + 8 && gx < kp.nn[0] - 1);
+                const bool shy = (gy == 2 * NY * ty && ty > 0) ||
+                                 (gy == 2 * NY * (ty + 1) && gy < kp.nn[1] - 1);
+                const bool shz = (t2 == 0 && k == 0 && tz > 0) || (t2 == 2 && gz < kp.nn[2] - 1);
+                const bool first = (!shx || (tx & 1) == 0) && (!shy || (ty & 1) == 0) &&
+                                   (!shz || (tz & 1) == 0);
+                T *d = dst + (gx + (int64_t)kp.nn[0] * (gy + (int64_t)kp.nn[1] * gz)) * NC;
+                if (!first)
+                  {
+                    T o[NC];
+                    q2_load_row<NC>(d, o);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                      v[c] = o[c] + v[c];
+                  }
+                q2_store_row<NC>(d, v);
+              }
+            else
+              keep = true;
+          }
+        __syncthreads();
+        if (keep)
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            carry[c * PXY + pn] = v[c];
+      }
   }
 
   // ------------------------------------------------ linearisation cache

@@ -2,7 +2,9 @@
 
 Same bar as the Q1 parity tests: FP64 relative l2 error <= 1e-12 for the
 vmult, residual, diagonal and node blocks; bitwise-identical repeats; FP32
-vmult per-field relative l2 error <= 1e-5.  Meshes mix odd and even cell
+vmult per-field relative l2 error <= 1e-5.  The 3D vmult runs the tile march
+by default; a test runs it at several march lengths against the colour
+passes.  Meshes mix odd and even cell
 counts and include a single cell along a direction, so every colour pass,
 empty colours and every first-colour rule of the scatter are exercised.
 """
@@ -109,6 +111,37 @@ def test_q2_residual_matches_oracle(dim, cells, G, tens, wtd):
     dst = op.initialize_dof_vector()
     op.evaluate_nonlinear_residual(dst, dev(u), wtd, old=[dev(o) for o in old])
     assert rel_l2(host(dst), ref) <= TOL64, rel_l2(host(dst), ref)
+
+
+@pytest.mark.parametrize("cells,G,tens", [
+    ((9, 6, 13), 2, False), ((5, 5, 7), 10, True), ((8, 3, 4), 12, True)])
+def test_q2_tile_march_matches_colour_passes(cells, G, tens, monkeypatch):
+    """3D vmult by the tile march (HPSINT_AMD_Q2_TILE cell layers per march)
+    against the colour passes (HPSINT_AMD_Q2_TILE=0) and the oracle, FP64 and
+    FP32; meshes with full and partial tiles along every direction."""
+    ops = {}
+    for zc in (0, 1, 2, 6, 64):
+        monkeypatch.setenv("HPSINT_AMD_Q2_TILE", str(zc))
+        ops[zc], p = make_pair(3, cells, G, tens, w0=100.0)
+    u = fields(p, 1)
+    src = np.random.default_rng(2).uniform(-1, 1, size=u.size)
+    ref = po.vmult(p, po.fill_qp(p, u), src)
+    out64, out32 = {}, {}
+    for zc, op in ops.items():
+        op.fill_quadrature_point_values(dev(u))
+        d = op.initialize_dof_vector()
+        op.vmult(d, dev(src))
+        out64[zc] = host(d)
+        d32 = op.initialize_dof_vector(torch.float32)
+        op.vmult(d32, dev(src, torch.float32))
+        out32[zc] = host(d32)
+        d2 = op.initialize_dof_vector()
+        op.vmult(d2, dev(src))
+        assert np.array_equal(host(d2), out64[zc])
+    for zc in ops:
+        assert rel_l2(out64[zc], ref) <= TOL64, (zc, rel_l2(out64[zc], ref))
+        assert rel_l2(out64[zc], out64[0]) <= 1e-13, zc
+        assert rel_l2(out32[zc], out32[0]) <= 1e-6, zc
 
 
 @pytest.mark.parametrize("dim,cells,G,tens", [
