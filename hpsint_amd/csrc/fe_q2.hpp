@@ -24,7 +24,9 @@
 //
 // Assembly.  Cells are processed in 2^DIM colour passes (parity of the cell
 // index along every direction); cells of one colour share no node, so a
-// pass updates dst without atomics.  The first colour that reaches a node
+// pass updates dst without atomics.  (The 3D tangent vmult instead marches
+// tiles of cells, q2_vmult_tile below, with the same cache and the same
+// per-wave cell body.)  The first colour that reaches a node
 // stores, later colours add, always in colour order: every entry is summed
 // in a fixed order (deterministic).  A cell is the first to reach its local
 // node (t0, t1, t2) iff along every direction its index is even, the node is
