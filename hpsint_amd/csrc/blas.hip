@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "blas.hpp"
 
@@ -490,27 +491,68 @@ namespace hps
         }
   }
 
+  // dst_n = B_n^-1 src_n for the nodes of a chunk of 256 (one thread per
+  // node).  The chunk's src and dst rows (256 NC consecutive values) move
+  // between HBM and LDS as 16-byte vectors, so the strided per-node rows
+  // are never read or written lane by lane from HBM; the inverse blocks are
+  // read entry-major (consecutive nodes, coalesced).  The chunk's src is
+  // read completely before any of its dst is written, so dst may be src.
   template <int NC, typename T>
-  __global__ void k_block_apply(int64_t n_nodes, const T *__restrict__ inv,
-                                const T *__restrict__ src, T *__restrict__ dst)
+  __global__ void __launch_bounds__(256)
+    k_block_apply(int64_t n_nodes, const T *__restrict__ inv, const T *src, T *dst)
   {
-    for (int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; node < n_nodes;
-         node += (int64_t)gridDim.x * blockDim.x)
+    constexpr int NB = 256;
+    constexpr int W  = 16 / (int)sizeof(T);
+    using V          = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    __shared__ T  buf[NB * NC];
+    const int64_t n0  = (int64_t)blockIdx.x * NB;
+    const int     nb  = n_nodes - n0 < NB ? (int)(n_nodes - n0) : NB;
+    const int     len = nb * NC;
+    const int     t   = (int)threadIdx.x;
+    const T      *s0  = src + n0 * NC;
+    T            *d0  = dst + n0 * NC;
+    const bool    vec = ((reinterpret_cast<uintptr_t>(s0) | reinterpret_cast<uintptr_t>(d0)) & 15) == 0;
+    const int     nv  = vec ? len / W : 0; // whole vectors; the rest by value
+    for (int v = t; v < nv; v += NB)
       {
-        T s[NC];
+        const V x = reinterpret_cast<const V *>(s0)[v];
+        __builtin_memcpy(buf + v * W, &x, 16);
+      }
+    for (int e = nv * W + t; e < len; e += NB)
+      buf[e] = s0[e];
+    __syncthreads();
+    T out[NC];
+    if (t < nb)
+      {
+        T x[NC];
 #pragma unroll
         for (int j = 0; j < NC; ++j)
-          s[j] = src[node * NC + j];
+          x[j] = buf[t * NC + j];
+        const T *ib = inv + n0 + t;
 #pragma unroll
         for (int i = 0; i < NC; ++i)
           {
             T acc = T(0);
 #pragma unroll
             for (int j = 0; j < NC; ++j)
-              acc += inv[(int64_t)(i * NC + j) * n_nodes + node] * s[j];
-            dst[node * NC + i] = acc;
+              acc += ib[(int64_t)(i * NC + j) * n_nodes] * x[j];
+            out[i] = acc;
           }
       }
+    __syncthreads();
+    if (t < nb)
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+        buf[t * NC + i] = out[i];
+    __syncthreads();
+    for (int v = t; v < nv; v += NB)
+      {
+        V x;
+        __builtin_memcpy(&x, buf + v * W, 16);
+        reinterpret_cast<V *>(d0)[v] = x;
+      }
+    for (int e = nv * W + t; e < len; e += NB)
+      d0[e] = buf[e];
   }
 
   __global__ void k_from_blocks(int64_t n_nodes, int nc, double *__restrict__ x,
@@ -742,7 +784,9 @@ namespace hps
   bool block_apply64(hipStream_t s, int64_t n_nodes, int nc, const double *inv,
                      const double *src, double *dst)
   {
-    const int g = grid_for(n_nodes, 256);
+    if (n_nodes <= 0)
+      return true;
+    const unsigned g = (unsigned)((n_nodes + 255) / 256);
 #define HPS_CALL(N) k_block_apply<N, double><<<g, 256, 0, s>>>(n_nodes, inv, src, dst)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
@@ -752,7 +796,9 @@ namespace hps
   bool block_apply32(hipStream_t s, int64_t n_nodes, int nc, const float *inv,
                      const float *src, float *dst)
   {
-    const int g = grid_for(n_nodes, 256);
+    if (n_nodes <= 0)
+      return true;
+    const unsigned g = (unsigned)((n_nodes + 255) / 256);
 #define HPS_CALL(N) k_block_apply<N, float><<<g, 256, 0, s>>>(n_nodes, inv, src, dst)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
