@@ -68,7 +68,7 @@ struct hps_operator_s
   hps::FixParams       fp;
   int                  n_tiles, tz;
   // 3D scalar tangent with z-column lanes (8 x 8 x tz tiles, ntz1 along y)
-  bool                 zcol, zcol_residual;
+  bool                 zcol, zcol32, zcol_residual;
   int                  ntz1, n_tiles_z;
   int64_t              n_nodes, n_dofs, cache_elems, part_elems;
   hipStream_t          stream;
@@ -295,11 +295,13 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
       // the z-column tangent (3D scalar mobility, up to ZCOL_MAX_G order
       // parameters, the cache of 8 x 4 tiles); HPSINT_AMD_ZCOL=0 keeps the
       // one-lane-per-corner kernel
-      op->zcol = prm->dim == 3 && !tens && G <= hps::ZCOL_MAX_G && ops->tx == hps::TX3 &&
-                 ops->ty == hps::TY3 && env_int("HPSINT_AMD_ZCOL", 1) != 0;
+      const bool zc = prm->dim == 3 && !tens && ops->tx == hps::TX3 && ops->ty == hps::TY3 &&
+                      env_int("HPSINT_AMD_ZCOL", 1) != 0;
+      op->zcol   = zc && G <= hps::ZCOL_MAX_G;
+      op->zcol32 = zc && G <= hps::ZCOL32_MAX_G;
       // the residual with z-column lanes as well (HPSINT_AMD_ZCOL_RES=0: not)
       op->zcol_residual = op->zcol && env_int("HPSINT_AMD_ZCOL_RES", 1) != 0;
-      if (op->zcol)
+      if (op->zcol || op->zcol32)
         {
           op->ntz1       = (ncy + 7) / 8;
           op->n_tiles_z  = kp.nt[0] * op->ntz1 * kp.nt[2];
@@ -716,7 +718,7 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
                             op->cut_masks, src, op->cache32, dst, (float *)op->part,
                             op->stream);
       }
-  else if (op->zcol && !op->cut_active)
+  else if (op->zcol32 && !op->cut_active)
     hps::vmult_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, src, op->cache32, dst,
                        (float *)op->part, op->stream);
   else
@@ -726,7 +728,7 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
   if (int r = finish_fixup<float>(op, dst, op->nc, 1, 0,
-                                  !op->cut_ready && op->zcol && !op->cut_active))
+                                  !op->cut_ready && op->zcol32 && !op->cut_active))
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));

@@ -204,6 +204,19 @@ namespace hps
     gyh         = Ayh + r * Ayl;
   }
 
+  // values only (same levels, same rounding as zcol_interp)
+  template <typename T>
+  __device__ __forceinline__ void zcol_interp_values(T ul, T uh, T &vl, T &vh)
+  {
+    const T r   = Q1Hat<T>::r;
+    const T Al  = ul + r * xshfl<1>(ul);
+    const T Ah  = uh + r * xshfl<1>(uh);
+    const T AAl = Al + r * xshfl<2>(Al);
+    const T AAh = Ah + r * xshfl<2>(Ah);
+    vl          = AAl + r * AAh;
+    vh          = AAh + r * AAl;
+  }
+
   // transpose of zcol_interp: the nodal contributions (lo, hi) of a row given
   // at the two Gauss points as (value, gradient) in the scaling of
   // integrate_vg_hat; levels z (registers), y, x
@@ -469,7 +482,12 @@ namespace hps
     // next layer's fields are requested during the current layer (PF); FP32:
     // at the top of their layer, which leaves the registers of the prefetch
     // free (3 % faster on the 128^3 workload; FP64 is 1 % faster with it)
-    constexpr bool PF = VM && sizeof(T) == 8;
+    // more than 4 order parameters (tangent): values first, the gradients of
+    // c and mu with them, an eta gradient interpolated again where its row
+    // is integrated (its registers are not held across the physics)
+    constexpr bool TWO = VM && G > 4;
+    constexpr int  NGV = TWO ? 2 : NC; // gradients interpolated with the values
+    constexpr bool PF  = VM && sizeof(T) == 8 && !TWO;
     T    lin_next[VM ? 2 : 1][VM ? NF : 1];
     auto load_lin = [&](int k) {
       if constexpr (VM)
@@ -570,11 +588,14 @@ namespace hps
                 lin[0][f] = lin_next[0][f];
                 lin[1][f] = lin_next[1][f];
               }
-            T        val[2][NC], gx[2][NC], gy[2][NC], gz[NC];
+            T        val[2][NC], gx[2][NGV], gy[2][NGV], gz[NGV];
 #pragma unroll
             for (int c = 0; c < NC; ++c)
-              zcol_interp(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c], gy[0][c],
-                          gy[1][c], gz[c]);
+              if (c < NGV)
+                zcol_interp(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c], gy[0][c],
+                            gy[1][c], gz[c]);
+              else
+                zcol_interp_values(plo[c], phi[c], val[0][c], val[1][c]);
             __builtin_amdgcn_sched_barrier(0);
 
             // value rows and the c-row mobility terms per Gauss point
@@ -582,14 +603,13 @@ namespace hps
 #pragma unroll
             for (int n = 0; n < 2; ++n)
               {
+                // (the value rows read no gradient)
                 T g[NC][3], gr_unused[NC][3];
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
-                  {
-                    g[c][0] = gx[n][c];
-                    g[c][1] = gy[n][c];
-                    g[c][2] = n ? -gz[c] : gz[c];
-                  }
+#pragma unroll
+                  for (int d = 0; d < 3; ++d)
+                    g[c][d] = T(0);
                 tangent_qp_rows<3, G, false, true, false>(kp.pp, lin[n], val[n], g, vr[n],
                                                           gr_unused);
                 scalar_mobility_terms<G>(kp.pp, lin[n][CL::c], lin[n] + CL::eta, val[n], M[n],
@@ -602,11 +622,26 @@ namespace hps
                 // the gradient a row needs: grad mu for the c row, grad c for
                 // the mu row, its own for an eta row
                 const int src_c = c == 0 ? 1 : (c == 1 ? 0 : c);
-                T         gs[2][3];
+                T         ex[2], ey[2], ez;
+                if (src_c < NGV)
+                  {
+                    ex[0] = gx[0][src_c];
+                    ex[1] = gx[1][src_c];
+                    ey[0] = gy[0][src_c];
+                    ey[1] = gy[1][src_c];
+                    ez    = gz[src_c];
+                  }
+                else
+                  {
+                    T vl_unused, vh_unused;
+                    zcol_interp(plo[c], phi[c], vl_unused, vh_unused, ex[0], ex[1], ey[0], ey[1],
+                                ez);
+                  }
+                T gs[2][3];
 #pragma unroll
                 for (int n = 0; n < 2; ++n)
                   {
-                    const T gg[3]  = {gx[n][src_c], gy[n][src_c], n ? -gz[src_c] : gz[src_c]};
+                    const T gg[3]  = {ex[n], ey[n], n ? -ez : ez};
                     const T gsj[3] = {fs.gsj[0], fs.gsj[1], n ? gsz : -gsz};
 #pragma unroll
                     for (int d = 0; d < 3; ++d)

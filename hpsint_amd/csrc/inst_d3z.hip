@@ -1,5 +1,5 @@
-// Instantiations: 3D Q1 scalar-mobility tangent with z-column lanes, 1-4
-// order parameters, FP64 and FP32.
+// Instantiations: 3D Q1 scalar-mobility tangent with z-column lanes, FP64
+// with 1-4 order parameters, FP32 with 1-12; the residual with 1-4.
 #include <cstdlib>
 
 #include "fe_q1_zcol.hpp"
@@ -8,14 +8,15 @@
 namespace hps
 {
   // waves per SIMD of the register allocation: HPSINT_AMD_ZCOL_WAVES (2, 3
-  // or 4) or the default
-  static int zcol_waves(bool fp64)
+  // or 4) or the default (FP64 2; FP32 3, or 2 with more than 4 order
+  // parameters, whose allocation spills at 3)
+  static int zcol_waves(bool fp64, int n_grains)
   {
     const char *e = std::getenv("HPSINT_AMD_ZCOL_WAVES");
     const int   w = e ? std::atoi(e) : 0;
     if (w >= 2 && w <= 4)
       return w;
-    return fp64 ? 2 : 3;
+    return fp64 || n_grains > 4 ? 2 : 3;
   }
 
   template <int G, typename T>
@@ -69,21 +70,42 @@ namespace hps
       }
   }
 
+  // the kernel of G order parameters, if built for T
+  template <int G, typename T>
+  static bool zcol_try(int w, const KParams &kp, int ntz1, int n_tiles, const T *src,
+                       const T *cache, T *dst, T *part, hipStream_t s)
+  {
+    if constexpr (G > (sizeof(T) == 8 ? ZCOL_MAX_G : ZCOL32_MAX_G))
+      return false;
+    else
+      {
+        zcol_kernel<G, T>(w, kp, ntz1, n_tiles, src, cache, dst, part, s);
+        return true;
+      }
+  }
+
   template <typename T>
   static bool zcol_launch(int n_grains, const KParams &kp, int ntz1, int n_tiles, const T *src,
                           const T *cache, T *dst, T *part, hipStream_t s)
   {
-    const int w = zcol_waves(sizeof(T) == 8);
+    const int w = zcol_waves(sizeof(T) == 8, n_grains);
     switch (n_grains)
       {
-#define HPS_CASE(G)                                                      \
-  case G:                                                                \
-    zcol_kernel<G, T>(w, kp, ntz1, n_tiles, src, cache, dst, part, s);   \
-    return true;
+#define HPS_CASE(G)                                                           \
+  case G:                                                                     \
+    return zcol_try<G, T>(w, kp, ntz1, n_tiles, src, cache, dst, part, s);
         HPS_CASE(1)
         HPS_CASE(2)
         HPS_CASE(3)
         HPS_CASE(4)
+        HPS_CASE(5)
+        HPS_CASE(6)
+        HPS_CASE(7)
+        HPS_CASE(8)
+        HPS_CASE(9)
+        HPS_CASE(10)
+        HPS_CASE(11)
+        HPS_CASE(12)
 #undef HPS_CASE
         default:
           return false;
@@ -109,6 +131,14 @@ namespace hps
         HPS_CASE(4)
         HPS_CASE(5)
         HPS_CASE(6)
+        HPS_CASE(7)
+        HPS_CASE(8)
+        HPS_CASE(9)
+        HPS_CASE(10)
+        HPS_CASE(11)
+        HPS_CASE(12)
+        HPS_CASE(13)
+        HPS_CASE(14)
 #undef HPS_CASE
         default:
           break;

@@ -306,7 +306,8 @@ namespace hps
   // the 3D scalar-mobility tangent with z-column lanes (fe_q1_zcol.hpp):
   // 8 x 8 x tz tiles, ntz1 of them along y; reads the cache of the TX3 x
   // TY3 tiles.  False when n_grains is not built for it.
-  constexpr int ZCOL_MAX_G = 4;
+  constexpr int ZCOL_MAX_G   = 4;  // FP64 tangent and the residual
+  constexpr int ZCOL32_MAX_G = 12; // FP32 tangent
   bool vmult_zcol_d3(int n_grains, const KParams &kp, int ntz1, int n_tiles, const double *src,
                      const double *cache, double *dst, double *part, hipStream_t s);
   bool vmult_zcol_d3(int n_grains, const KParams &kp, int ntz1, int n_tiles, const float *src,

@@ -478,11 +478,13 @@ def test_every_grain_count_is_built():
 
 @pytest.mark.parametrize("zcol", ["0", "1"])
 @pytest.mark.parametrize("cells,G", [((19, 13, 37), 4), ((9, 5, 17), 2), ((3, 7, 2), 3),
-                                     ((8, 12, 16), 1), ((17, 9, 33), 4)])
+                                     ((8, 12, 16), 1), ((17, 9, 33), 4), ((11, 9, 18), 6),
+                                     ((19, 9, 20), 8), ((10, 17, 9), 12)])
 def test_vmult_zcol_and_corner_lanes(monkeypatch, zcol, cells, G):
-    """the 3D scalar tangent with z-column lanes (8 x 8 tiles) and with one
-    lane per cell corner give the oracle's vmult, FP64 at 1e-12 and FP32 per
-    field at 1e-5"""
+    """the 3D scalar tangent with z-column lanes (8 x 8 tiles; FP64 with up
+    to 4 order parameters, FP32 with up to 12) and with one lane per cell
+    corner give the oracle's vmult, FP64 at 1e-12 and FP32 per field at
+    1e-5"""
     monkeypatch.setenv("HPSINT_AMD_ZCOL", zcol)
     op, p = make_pair(3, cells, G, False, w0=100.0)
     u = fields(p, 3)
