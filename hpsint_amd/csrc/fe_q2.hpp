@@ -496,6 +496,16 @@ namespace hps
     constexpr int NF = CL::n_fields;
     const int     l  = L.l;
 
+    // the tensorial Q_i fields are requested before the interpolation
+    // passes, so that they arrive while the passes run
+    constexpr bool EARLYQ = TENS && G > 1;
+    T              qf[EARLYQ ? G * DIM : 1];
+    if constexpr (EARLYQ)
+      {
+#pragma unroll
+        for (int f = 0; f < G * DIM; ++f)
+          qf[f] = L.valid ? cl[(CacheLayout<DIM, G, TENS>::gb_q + f) * Q2Lane<DIM>::LP] : T(0);
+      }
     // 1) values at the Gauss points, one pass per direction for all
     //    components (U -> A -> U -> registers in 3D, U -> A -> registers in 2D)
     T v[NC];
@@ -555,7 +565,7 @@ namespace hps
       T vr[NC], gc0[DIM];
       if constexpr (TENS && G > 1)
         {
-          // sum_i deta_i Q_i streamed from the cache first
+          // sum_i deta_i Q_i from the Q_i fields requested at the top
           T qsum[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; ++d)
@@ -564,7 +574,7 @@ namespace hps
           for (int i = 0; i < G; ++i)
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-              qsum[d] += v[2 + i] * (L.valid ? cl[(CL::gb_q + i * DIM + d) * LP] : T(0));
+              qsum[d] += v[2 + i] * qf[i * DIM + d];
           T lin[CL::gb_q];
 #pragma unroll
           for (int f = 0; f < CL::gb_q; ++f)
