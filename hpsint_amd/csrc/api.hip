@@ -843,8 +843,9 @@ static int precond_update(hps_operator op, int kind, bool fp32)
       if (!op->blocks)
         {
           HPS_HIP(hipMalloc(&op->blocks, nn * nc * nc * sizeof(double)));
-          HPS_HIP(hipMalloc(&op->inv_blocks64, nn * nc * nc * sizeof(double)));
-          HPS_HIP(hipMalloc(&op->inv_blocks32, nn * nc * nc * sizeof(float)));
+          const int64_t ni = hps::inv_stride(nn) * nc * nc;
+          HPS_HIP(hipMalloc(&op->inv_blocks64, ni * sizeof(double)));
+          HPS_HIP(hipMalloc(&op->inv_blocks32, ni * sizeof(float)));
         }
       // the assembled blocks are stored per node column by column
       // (blocks[node][column][row]): a block column is a contiguous run

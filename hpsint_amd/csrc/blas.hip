@@ -322,7 +322,7 @@ namespace hps
   // do_update, I/lac/preconditioners.h:288-308)
   template <int NC>
   __global__ void __launch_bounds__(64)
-    k_invert_blocks(int64_t n_nodes, const double *__restrict__ blocks,
+    k_invert_blocks(int64_t n_nodes, int64_t ist, const double *__restrict__ blocks,
                                   double *__restrict__ inv64, float *__restrict__ inv32,
                                   int *__restrict__ singular)
   {
@@ -397,8 +397,8 @@ namespace hps
     for (int i = 0; i < NC; ++i)
       for (int j = 0; j < NC; ++j)
         {
-          inv64[(int64_t)(i * NC + j) * n_nodes + node] = b[i][j];
-          inv32[(int64_t)(i * NC + j) * n_nodes + node] = (float)b[i][j];
+          inv64[(int64_t)(i * NC + j) * ist + node] = b[i][j];
+          inv32[(int64_t)(i * NC + j) * ist + node] = (float)b[i][j];
         }
   }
 
@@ -412,7 +412,7 @@ namespace hps
   // 16 consecutive nodes)
   template <int NC>
   __global__ void __launch_bounds__(256)
-    k_invert_blocks_rows(int64_t n_nodes, const double *__restrict__ blocks,
+    k_invert_blocks_rows(int64_t n_nodes, int64_t ist, const double *__restrict__ blocks,
                          double *__restrict__ inv64, float *__restrict__ inv32,
                          int *__restrict__ singular)
   {
@@ -486,24 +486,29 @@ namespace hps
 #pragma unroll
       for (int k = 0; k < NC; ++k)
         {
-          inv64[(int64_t)(col * NC + k) * n_nodes + node] = b[k];
-          inv32[(int64_t)(col * NC + k) * n_nodes + node] = (float)b[k];
+          inv64[(int64_t)(col * NC + k) * ist + node] = b[k];
+          inv32[(int64_t)(col * NC + k) * ist + node] = (float)b[k];
         }
   }
 
-  // dst_n = B_n^-1 src_n for the nodes of a chunk of 256 (one thread per
-  // node).  The chunk's src and dst rows (256 NC consecutive values) move
-  // between HBM and LDS as 16-byte vectors, so the strided per-node rows
-  // are never read or written lane by lane from HBM; the inverse blocks are
-  // read entry-major (consecutive nodes, coalesced).  The chunk's src is
-  // read completely before any of its dst is written, so dst may be src.
-  template <int NC, typename T>
+  // dst_n = B_n^-1 src_n for the nodes of a chunk of 256 NPT, thread t
+  // taking the NPT adjacent nodes NPT t ...  The chunk's src and dst rows
+  // (256 NPT NC consecutive values) move between HBM and LDS as 16-byte
+  // vectors, so the strided per-node rows are never read or written lane by
+  // lane from HBM; the inverse blocks are read entry-major (entry e of node
+  // n at e ist + n, ist a multiple of 4), the NPT nodes of a lane in one
+  // vector access, so that one access of a wave covers 256 NPT consecutive
+  // values of an entry.  The chunk's src is read completely before any of
+  // its dst is written, so dst may be src.
+  template <int NC, typename T, int NPT>
   __global__ void __launch_bounds__(256)
-    k_block_apply(int64_t n_nodes, const T *__restrict__ inv, const T *src, T *dst)
+    k_block_apply(int64_t n_nodes, int64_t ist, const T *__restrict__ inv, const T *src, T *dst)
   {
-    constexpr int NB = 256;
+    static_assert(NPT == 1 || NPT == 2, "one or two nodes per thread");
+    constexpr int NB = 256 * NPT;
     constexpr int W  = 16 / (int)sizeof(T);
     using V          = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    using P          = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     __shared__ T  buf[NB * NC];
     const int64_t n0  = (int64_t)blockIdx.x * NB;
     const int     nb  = n_nodes - n0 < NB ? (int)(n_nodes - n0) : NB;
@@ -513,45 +518,65 @@ namespace hps
     T            *d0  = dst + n0 * NC;
     const bool    vec = ((reinterpret_cast<uintptr_t>(s0) | reinterpret_cast<uintptr_t>(d0)) & 15) == 0;
     const int     nv  = vec ? len / W : 0; // whole vectors; the rest by value
-    for (int v = t; v < nv; v += NB)
+    for (int v = t; v < nv; v += 256)
       {
         const V x = reinterpret_cast<const V *>(s0)[v];
         __builtin_memcpy(buf + v * W, &x, 16);
       }
-    for (int e = nv * W + t; e < len; e += NB)
+    for (int e = nv * W + t; e < len; e += 256)
       buf[e] = s0[e];
     __syncthreads();
-    T out[NC];
-    if (t < nb)
+    const int tn = NPT * t; // first node of the thread in the chunk
+    T         out[NPT][NC];
+    if (tn < nb)
       {
-        T x[NC];
+        T x[NPT][NC];
 #pragma unroll
-        for (int j = 0; j < NC; ++j)
-          x[j] = buf[t * NC + j];
-        const T *ib = inv + n0 + t;
+        for (int p = 0; p < NPT; ++p)
+#pragma unroll
+          for (int j = 0; j < NC; ++j)
+            x[p][j] = buf[(tn + p) * NC + j]; // the chunk's rows past nb hold junk: unused
+        const T *ib = inv + n0 + tn;
 #pragma unroll
         for (int i = 0; i < NC; ++i)
           {
-            T acc = T(0);
+            T acc[NPT];
+#pragma unroll
+            for (int p = 0; p < NPT; ++p)
+              acc[p] = T(0);
 #pragma unroll
             for (int j = 0; j < NC; ++j)
-              acc += ib[(int64_t)(i * NC + j) * n_nodes] * x[j];
-            out[i] = acc;
+              {
+                const T *e = ib + (int64_t)(i * NC + j) * ist;
+                if constexpr (NPT == 1)
+                  acc[0] += e[0] * x[0][j];
+                else
+                  {
+                    const P y = *reinterpret_cast<const P *>(e);
+                    acc[0] += y.x * x[0][j];
+                    acc[1] += y.y * x[1][j];
+                  }
+              }
+#pragma unroll
+            for (int p = 0; p < NPT; ++p)
+              out[p][i] = acc[p];
           }
       }
     __syncthreads();
-    if (t < nb)
 #pragma unroll
-      for (int i = 0; i < NC; ++i)
-        buf[t * NC + i] = out[i];
+    for (int p = 0; p < NPT; ++p)
+      if (tn + p < nb)
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+          buf[(tn + p) * NC + i] = out[p][i];
     __syncthreads();
-    for (int v = t; v < nv; v += NB)
+    for (int v = t; v < nv; v += 256)
       {
         V x;
         __builtin_memcpy(&x, buf + v * W, 16);
         reinterpret_cast<V *>(d0)[v] = x;
       }
-    for (int e = nv * W + t; e < len; e += NB)
+    for (int e = nv * W + t; e < len; e += 256)
       d0[e] = buf[e];
   }
 
@@ -771,11 +796,12 @@ namespace hps
     // up to 6 components both matrices stay in one lane's registers
     const int g  = (int)((n_nodes + 63) / 64);
     const int g16 = (int)((n_nodes + 15) / 16);
+    const int64_t ist = inv_stride(n_nodes);
 #define HPS_CALL(N)                                                                          \
   if constexpr (N <= 6)                                                                      \
-    k_invert_blocks<N><<<g, 64, 0, s>>>(n_nodes, blocks, inv64, inv32, singular);            \
+    k_invert_blocks<N><<<g, 64, 0, s>>>(n_nodes, ist, blocks, inv64, inv32, singular);       \
   else                                                                                       \
-    k_invert_blocks_rows<N><<<g16, 256, 0, s>>>(n_nodes, blocks, inv64, inv32, singular)
+    k_invert_blocks_rows<N><<<g16, 256, 0, s>>>(n_nodes, ist, blocks, inv64, inv32, singular)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
     return true;
@@ -787,7 +813,8 @@ namespace hps
     if (n_nodes <= 0)
       return true;
     const unsigned g = (unsigned)((n_nodes + 255) / 256);
-#define HPS_CALL(N) k_block_apply<N, double><<<g, 256, 0, s>>>(n_nodes, inv, src, dst)
+#define HPS_CALL(N) \
+  k_block_apply<N, double, 1><<<g, 256, 0, s>>>(n_nodes, inv_stride(n_nodes), inv, src, dst)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
     return true;
@@ -798,8 +825,16 @@ namespace hps
   {
     if (n_nodes <= 0)
       return true;
-    const unsigned g = (unsigned)((n_nodes + 255) / 256);
-#define HPS_CALL(N) k_block_apply<N, float><<<g, 256, 0, s>>>(n_nodes, inv, src, dst)
+    // two nodes per thread from 9 components on (one access of a wave then
+    // covers 512 consecutive values of an entry); below, the 512-node
+    // chunks leave too few workgroups and one node per thread is faster
+#define HPS_CALL(N)                                                                       \
+  if constexpr (N >= 9)                                                                   \
+    k_block_apply<N, float, 2><<<(unsigned)((n_nodes + 511) / 512), 256, 0, s>>>(         \
+      n_nodes, inv_stride(n_nodes), inv, src, dst);                                       \
+  else                                                                                    \
+    k_block_apply<N, float, 1><<<(unsigned)((n_nodes + 255) / 256), 256, 0, s>>>(         \
+      n_nodes, inv_stride(n_nodes), inv, src, dst)
     HPS_NC_SWITCH(nc, HPS_CALL)
 #undef HPS_CALL
     return true;

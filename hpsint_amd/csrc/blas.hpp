@@ -29,6 +29,13 @@ namespace hps
   void diag_apply64(hipStream_t, int64_t n, const double *inv, const double *src, double *dst);
   void diag_apply32(hipStream_t, int64_t n, const float *inv, const float *src, float *dst);
   void unit_empty_rows(hipStream_t, int64_t n_nodes, int nc, double *blocks);
+  // node-block inverses are stored entry-major with a node stride of
+  // inv_stride(n_nodes) (a multiple of 4): entry e of node n at e * stride + n;
+  // a buffer of them holds inv_stride(n_nodes) * nc * nc values
+  inline int64_t inv_stride(int64_t n_nodes)
+  {
+    return (n_nodes + 3) & ~(int64_t)3;
+  }
   bool invert_blocks(hipStream_t, int64_t n_nodes, int nc, const double *blocks, double *inv64,
                      float *inv32, int *singular);
   bool block_apply64(hipStream_t, int64_t n_nodes, int nc, const double *inv, const double *src,
