@@ -846,6 +846,10 @@ static int precond_update(hps_operator op, int kind, bool fp32)
           const int64_t ni = hps::inv_stride(nn) * nc * nc;
           HPS_HIP(hipMalloc(&op->inv_blocks64, ni * sizeof(double)));
           HPS_HIP(hipMalloc(&op->inv_blocks32, ni * sizeof(float)));
+          // the padding nodes past nn are read (two nodes per lane in the
+          // FP32 apply; their results are dropped) but never written: zeros
+          HPS_HIP(hipMemsetAsync(op->inv_blocks64, 0, ni * sizeof(double), op->stream));
+          HPS_HIP(hipMemsetAsync(op->inv_blocks32, 0, ni * sizeof(float), op->stream));
         }
       // the assembled blocks are stored per node column by column
       // (blocks[node][column][row]): a block column is a contiguous run
