@@ -846,8 +846,10 @@ static int precond_update(hps_operator op, int kind, bool fp32)
           const int64_t ni = hps::inv_stride(nn) * nc * nc;
           HPS_HIP(hipMalloc(&op->inv_blocks64, ni * sizeof(double)));
           HPS_HIP(hipMalloc(&op->inv_blocks32, ni * sizeof(float)));
-          // the padding nodes past nn are read (two nodes per lane in the
-          // FP32 apply; their results are dropped) but never written: zeros
+          // the padding nodes past nn are read by the FP32 apply (two nodes
+          // per lane) and their results dropped, so their values do not
+          // matter (the FP32 assembly stages its blocks in inv_blocks32);
+          // they start as zeros
           HPS_HIP(hipMemsetAsync(op->inv_blocks64, 0, ni * sizeof(double), op->stream));
           HPS_HIP(hipMemsetAsync(op->inv_blocks32, 0, ni * sizeof(float), op->stream));
         }
