@@ -887,8 +887,8 @@ static int precond_update(hps_operator op, int kind, bool fp32)
       if (fp32)
         hps::blas_f32_to_f64(op->stream, nn * nc * nc, op->blocks, op->inv_blocks32);
       // cut-off: unit diagonal for the components without any entry
-      if (op->cut_active)
-        hps::unit_empty_rows(op->stream, nn, nc, op->blocks);
+      if (op->cut_active && !hps::unit_empty_rows(op->stream, nn, nc, op->blocks))
+        return fail(HPS_ERR_INVALID_N_COMP, "node-block Jacobi not built for this n_components");
       // identity rows and columns of the constrained nodes: identity blocks
       if (op->n_constrained)
         hps::constraint_identity_blocks(op->stream, op->n_constrained, nc, op->constrained,

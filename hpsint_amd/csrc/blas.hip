@@ -767,27 +767,49 @@ namespace hps
   // grain cut-off: a component with no entry in its node block's row and
   // column (an order parameter dropped in every cell around the node) gets a
   // unit diagonal, the rule of the point-Jacobi diagonal (|d| <= 1e-10 -> 1,
-  // S/operator_base.h:364-389); blocks stored column by column per node
-  __global__ void k_unit_empty_rows(int64_t n_nodes, int nc, double *__restrict__ blocks)
+  // S/operator_base.h:364-389); blocks stored column by column per node.
+  // One wave per node: lane l reads entries l, l + 64, ... of the node's
+  // block (coalesced), marks the row and the column of each nonzero entry
+  // in a bit mask, the masks are OR-reduced over the wave, and lane a sets
+  // the diagonal of an order parameter a whose row and column are empty.
+  template <int NC>
+  __global__ void __launch_bounds__(256) k_unit_empty_rows(int64_t n_nodes, double *blocks)
   {
-    for (int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; node < n_nodes;
-         node += (int64_t)gridDim.x * blockDim.x)
+    static_assert(NC <= 32, "row/column bits in one mask");
+    constexpr int NE   = NC * NC;
+    constexpr int PER  = (NE + 63) / 64;
+    const int     lane = (int)threadIdx.x & 63;
+    const int64_t nw   = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; node < n_nodes;
+         node += nw)
       {
-        double *b = blocks + node * nc * nc;
-        for (int a = 2; a < nc; ++a)
+        double  *b = blocks + node * NE;
+        unsigned m = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
           {
-            bool empty = true;
-            for (int j = 0; j < nc && empty; ++j)
-              empty = b[j * nc + a] == 0.0 && b[a * nc + j] == 0.0;
-            if (empty)
-              b[a * nc + a] = 1.0;
+            const int e = lane + 64 * k;
+            if (e < NE && b[e] != 0.0)
+              {
+                const int j = e / NC, i = e - j * NC; // column j, row i
+                m |= (1u << i) | (1u << j);
+              }
           }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+          m |= (unsigned)__shfl_xor((int)m, o, 64);
+        if (lane >= 2 && lane < NC && ((m >> lane) & 1u) == 0)
+          b[lane * NC + lane] = 1.0;
       }
   }
 
-  void unit_empty_rows(hipStream_t s, int64_t n_nodes, int nc, double *blocks)
+  bool unit_empty_rows(hipStream_t s, int64_t n_nodes, int nc, double *blocks)
   {
-    k_unit_empty_rows<<<grid_for(n_nodes, 256), 256, 0, s>>>(n_nodes, nc, blocks);
+    const int g = grid_for(n_nodes * 64, 256);
+#define HPS_CALL(N) k_unit_empty_rows<N><<<g, 256, 0, s>>>(n_nodes, blocks)
+    HPS_NC_SWITCH(nc, HPS_CALL)
+#undef HPS_CALL
+    return true;
   }
 
   bool invert_blocks(hipStream_t s, int64_t n_nodes, int nc, const double *blocks,

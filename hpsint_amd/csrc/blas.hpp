@@ -28,7 +28,7 @@ namespace hps
   void invert_diag(hipStream_t, int64_t n, const double *d, double *inv64, float *inv32);
   void diag_apply64(hipStream_t, int64_t n, const double *inv, const double *src, double *dst);
   void diag_apply32(hipStream_t, int64_t n, const float *inv, const float *src, float *dst);
-  void unit_empty_rows(hipStream_t, int64_t n_nodes, int nc, double *blocks);
+  bool unit_empty_rows(hipStream_t, int64_t n_nodes, int nc, double *blocks);
   // node-block inverses are stored entry-major with a node stride of
   // inv_stride(n_nodes) (a multiple of 4): entry e of node n at e * stride + n;
   // a buffer of them holds inv_stride(n_nodes) * nc * nc values
