@@ -758,6 +758,7 @@ static hps::ChParams ch_params(hps_operator op)
   cp.n_nodes  = op->n_nodes;
   cp.n_comp   = op->nc;
   cp.G        = op->n_grains;
+  cp.tens     = op->tens ? 1 : 0;
   cp.tx       = op->ops->tx;
   cp.ty       = op->ops->ty;
   cp.tz       = op->kp.tz;
@@ -781,9 +782,8 @@ static int precond_update(hps_operator op, int kind, bool fp32)
   const int     nc = op->nc;
   if (kind == HPS_PRECON_BLOCK_CH_ILU)
     {
-      if (op->q2 || op->tens)
-        return fail(HPS_ERR_UNSUPPORTED,
-                    "the Cahn-Hilliard ILU preconditioner is built for Q1, scalar mobility");
+      if (op->q2)
+        return fail(HPS_ERR_UNSUPPORTED, "the Cahn-Hilliard ILU preconditioner is built for Q1");
       if (op->n_constrained)
         return fail(HPS_ERR_UNSUPPORTED,
                     "the Cahn-Hilliard ILU preconditioner takes no constrained nodes");

@@ -202,6 +202,161 @@ namespace hps
       }
   }
 
+  // Tensorial mobility (S/mobility.h:758-916): the c row's flux is linear
+  // in (delta c, grad delta c, grad delta mu) with Gauss-point coefficients
+  // b (of delta c), Dc (of grad delta c: the projector derivative) and Mt
+  // (of grad delta mu: the mobility tensor).  One thread per (cell, corner
+  // k): thread k finds the coefficients of Gauss point k by evaluating the
+  // tangent's c row (tangent_qp_rows) at unit inputs, shares them through
+  // LDS, and adds the element-matrix row of its corner:
+  //   (c, c):   JxW (w0 phi_k phi_m + (b phi_m + Dc grad phi_m) . grad phi_k)
+  //   (c, mu):  JxW (Mt grad phi_m) . grad phi_k
+  //   (mu, c), (mu, mu) as with scalar mobility.
+  template <int DIM, int G>
+  __global__ void __launch_bounds__(256)
+    k_ch_assemble_tens(const ChParams cp, int par, const double *__restrict__ cache,
+                       double *__restrict__ A)
+  {
+    using CL            = CacheLayout<DIM, G, true>;
+    constexpr int NV    = 1 << DIM;
+    constexpr int NF    = CL::n_fields;
+    constexpr int CPB   = 256 / NV; // cells per workgroup
+    constexpr int NCOEF = DIM + 2 * DIM * DIM + 1;
+    __shared__ double coef[CPB][NV][NCOEF]; // b, Dc, Mt (row-major), d2f/dc2
+    const int     px = par & 1, py = (par >> 1) & 1, pz = (par >> 2) & 1;
+    const int     mx = (cp.nc[0] - px + 1) / 2, my = (cp.nc[1] - py + 1) / 2;
+    const int     mz = DIM == 3 ? (cp.nc[2] - pz + 1) / 2 : 1;
+    const int     lc = (int)threadIdx.x / NV, k = (int)threadIdx.x % NV;
+    const int64_t i  = (int64_t)blockIdx.x * CPB + lc;
+    const bool    valid = i < (int64_t)mx * my * mz;
+    const int     cx = px + 2 * (int)(i % mx);
+    const int     cy = py + 2 * (int)((i / mx) % my);
+    const int     cz = DIM == 3 ? pz + 2 * (int)(i / ((int64_t)mx * my)) : 0;
+    if (valid)
+      {
+        // the cell's place in the cache (as k_ch_assemble)
+        const int     cm    = DIM == 3 ? cz : cy;
+        const int     cyy   = DIM == 3 ? cy : 0;
+        const int64_t tile  = (int64_t)(cx / cp.tx) +
+                             (int64_t)cp.nt[0] * ((cyy / cp.ty) + (int64_t)cp.nt[1] * (cm / cp.tz));
+        const int     layer = cm % cp.tz;
+        const int     lane  = ((((cyy % cp.ty) * cp.tx) + (cx % cp.tx)) << DIM) + k;
+        const double *cl    = cache + ((tile * cp.tz + layer) * cp.nf) * (int64_t)cp.nt_lanes;
+        double        lin[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          lin[f] = cl[f * (int64_t)cp.nt_lanes + lane];
+        double v[G + 2], g[G + 2][DIM], vr[G + 2], gr[G + 2][DIM];
+#pragma unroll
+        for (int c = 0; c < G + 2; ++c)
+          {
+            v[c] = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              g[c][d] = 0.0;
+          }
+        double *out = coef[lc][k];
+        v[0]        = 1.0;
+        tangent_qp_rows<DIM, G, true, false, true>(cp.pp, lin, v, g, vr, gr);
+        v[0] = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+          out[d] = gr[0][d];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e)
+          {
+            g[0][e] = 1.0;
+            tangent_qp_rows<DIM, G, true, false, true>(cp.pp, lin, v, g, vr, gr);
+            g[0][e] = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              out[DIM + d * DIM + e] = gr[0][d];
+            g[1][e] = 1.0;
+            tangent_qp_rows<DIM, G, true, false, true>(cp.pp, lin, v, g, vr, gr);
+            g[1][e] = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              out[DIM + DIM * DIM + d * DIM + e] = gr[0][d];
+          }
+        const double c = lin[CL::c], A_ = cp.pp.A, B_ = cp.pp.B;
+        out[DIM + 2 * DIM * DIM] = 2.0 * A_ * (c * c) + 4.0 * A_ * c * (2.0 * c - 2.0) +
+                                   2.0 * A_ * ((1.0 - c) * (1.0 - c)) + 2.0 * B_;
+      }
+    __syncthreads();
+    if (!valid)
+      return;
+    constexpr double sa  = 0.78867513459481288225457439025098;
+    constexpr double sb  = 0.21132486540518711774542560974902;
+    const double     jxw = cp.jxw;
+    const int        kx = cx + (k & 1), ky = cy + ((k >> 1) & 1);
+    const int        kz = DIM == 3 ? cz + ((k >> 2) & 1) : 0;
+    // the colour of corner k: the cell parity flipped by the corner's bits
+    // (uniform only per corner, so the row is addressed per lane here)
+    const int            col = par ^ k;
+    const int64_t        ic  = (kx >> 1) + (int64_t)cp.mx[col] * ((ky >> 1) + (int64_t)cp.my[col] * (kz >> 1));
+    double              *rb  = A + cp.off[col] + 4 * ic;
+    const int64_t        rs  = 4 * cp.cnt[col];
+#pragma unroll 1
+    for (int m = 0; m < NV; ++m)
+      {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+          {
+            const double *cf = coef[lc][q];
+            double        vk = 1.0, vm = 1.0, gk[DIM], gm[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              {
+                vk *= (((k >> d) & 1) == ((q >> d) & 1)) ? sa : sb;
+                vm *= (((m >> d) & 1) == ((q >> d) & 1)) ? sa : sb;
+              }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              {
+                double fk = ((k >> d) & 1) ? 1.0 : -1.0, fm = ((m >> d) & 1) ? 1.0 : -1.0;
+#pragma unroll
+                for (int e = 0; e < DIM; ++e)
+                  if (e != d)
+                    {
+                      fk *= (((k >> e) & 1) == ((q >> e) & 1)) ? sa : sb;
+                      fm *= (((m >> e) & 1) == ((q >> e) & 1)) ? sa : sb;
+                    }
+                gk[d] = fk * cp.inv_h[d];
+                gm[d] = fm * cp.inv_h[d];
+              }
+            double gg = 0.0, cc = 0.0, cm = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              {
+                gg += gm[d] * gk[d];
+                double fc = cf[d] * vm, fmu = 0.0;
+#pragma unroll
+                for (int e = 0; e < DIM; ++e)
+                  {
+                    fc += cf[DIM + d * DIM + e] * gm[e];
+                    fmu += cf[DIM + DIM * DIM + d * DIM + e] * gm[e];
+                  }
+                cc += fc * gk[d];
+                cm += fmu * gk[d];
+              }
+            const double vv = vm * vk;
+            a0 += cp.pp.w0 * vv + cc;
+            a1 += cm;
+            a2 += -cf[DIM + 2 * DIM * DIM] * vv - cp.pp.kappa_c * gg;
+            a3 += cp.pp.w_alg * vv;
+          }
+        const int dx = (m & 1) - (k & 1), dy = ((m >> 1) & 1) - ((k >> 1) & 1);
+        const int dz = DIM == 3 ? ((m >> 2) & 1) - ((k >> 2) & 1) : 0;
+        const int s  = Sten<DIM>::index(dx, dy, dz);
+        double   *e4 = rb + s * rs;
+        e4[0] += jxw * a0;
+        e4[1] += jxw * a1;
+        e4[2] += jxw * a2;
+        e4[3] += jxw * a3;
+      }
+  }
+
   // a node of colour COL with in-colour index ic: its place in the colour's
   // grid (ix, iy, iz), its coordinates and its node number
   template <int DIM, int COL>
@@ -568,6 +723,44 @@ namespace hps
       }
   }
 
+  template <int DIM>
+  static void ch_assemble_tens_dim(const ChParams &cp, int par, int64_t n, const double *cache,
+                                   double *A, hipStream_t s)
+  {
+    const unsigned b = (unsigned)((n + (256 >> DIM) - 1) / (256 >> DIM));
+    switch (cp.G)
+      {
+#define HPS_CASE(G)                                                                \
+  case G:                                                                          \
+    k_ch_assemble_tens<DIM, G><<<b, 256, 0, s>>>(cp, par, cache, A);               \
+    break;
+        HPS_CASE(1)
+        HPS_CASE(2)
+        HPS_CASE(3)
+        HPS_CASE(4)
+        HPS_CASE(5)
+        HPS_CASE(6)
+        HPS_CASE(7)
+        HPS_CASE(8)
+        HPS_CASE(9)
+        HPS_CASE(10)
+        HPS_CASE(11)
+        HPS_CASE(12)
+#undef HPS_CASE
+        default:
+          break;
+      }
+  }
+
+  static void ch_assemble_tens(const ChParams &cp, int par, int64_t n, const double *cache,
+                               double *A, hipStream_t s)
+  {
+    if (cp.dim == 3)
+      ch_assemble_tens_dim<3>(cp, par, n, cache, A, s);
+    else
+      ch_assemble_tens_dim<2>(cp, par, n, cache, A, s);
+  }
+
   void ch_assemble(const ChParams &cp, const double *cache, double *A, hipStream_t s)
   {
     const int64_t nA = cp.n_nodes * (cp.dim == 3 ? 27 : 9) * 4;
@@ -580,7 +773,9 @@ namespace hps
         const int64_t n  = mx * my * mz;
         if (n == 0)
           continue;
-        if (cp.dim == 3)
+        if (cp.tens)
+          ch_assemble_tens(cp, par, n, cache, A, s);
+        else if (cp.dim == 3)
           k_ch_assemble<3><<<blocks_for(n), 256, 0, s>>>(cp, par, cache, A);
         else
           k_ch_assemble<2><<<blocks_for(n), 256, 0, s>>>(cp, par, cache, A);

@@ -18,6 +18,7 @@ namespace hps
     int        nn[3];      // nodes per physical direction (nn[2] = 1 in 2D)
     int64_t    n_nodes;
     int        n_comp, G;
+    int        tens;       // tensorial mobility
     // the Q1 linearisation cache: tiles of tx x ty cells and tz layers along
     // the last direction, nt tiles per direction (x, y (1 in 2D), march),
     // nt_lanes lanes per tile layer, nf fields per q-point
@@ -45,7 +46,7 @@ namespace hps
   }
 
   // Cahn-Hilliard block of the tangent (FP64) from the FP64
-  // linearisation cache (scalar mobility)
+  // linearisation cache (scalar or tensorial mobility, 1-12 order parameters)
   void ch_assemble(const ChParams &cp, const double *cache, double *A, hipStream_t s);
   // in-place ILU(0) (FP64) and its FP32 copy
   void ch_factor(const ChParams &cp, double *A, float *A32, hipStream_t s);

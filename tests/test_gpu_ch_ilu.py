@@ -1,9 +1,9 @@
 """The GPU's BlockPreconditioner2 (ILU(0) of the Cahn-Hilliard block in
 parity-colour order, inverse diagonal on the order parameters) against its
 restatement on the oracle's tangent (tests/precond_study.py): FP64 apply at
-1e-12, FP32 apply at 1e-5, on random fields and on the 8-sphere packing's
-fields at the 128^3 cell size; and a Newton step with it satisfies the
-oracle's residual."""
+1e-12, FP32 apply at 1e-5, on random fields (scalar and tensorial
+mobility) and on the 8-sphere packing's fields at the 128^3 cell size; and
+a Newton step with it satisfies the oracle's residual."""
 import numpy as np
 import pytest
 import torch
@@ -31,12 +31,12 @@ def host(t):
     return t.detach().cpu().numpy().astype(np.float64)
 
 
-def check(mesh, G, u, dt=1e-3, seed=3):
+def check(mesh, G, u, dt=1e-3, seed=3, tens=False):
     dim = mesh.dim
-    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(), MobilityParams())
+    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(), MobilityParams(tensorial=tens))
     op.set_time_weights(1.0 / dt, (-1.0 / dt,))
-    p = po.make_params(dim, 1, mesh.n_cells, mesh.cell_size, G + 2, w0=1.0 / dt,
-                       w_hist=(-1.0 / dt,))
+    p = po.make_params(dim, 1, mesh.n_cells, mesh.cell_size, G + 2, tensorial=tens,
+                       w0=1.0 / dt, w_hist=(-1.0 / dt,))
     A = ps.sparse_tangent(p, po.fill_qp(p, u), keep_zeros=True)
     ref_apply = ps.parity_colour_ilu(A, G + 2, [n + 1 for n in mesh.n_cells])
     op.fill_quadrature_point_values(dev(u))
@@ -70,6 +70,21 @@ def test_ch_ilu_random_fields(cells, G):
     u = rng.uniform(0, 1, size=(nn, G + 2))
     u[:, 1] = rng.uniform(-1, 1, nn)
     check(mesh, G, u.ravel())
+
+
+@pytest.mark.parametrize("cells,G", [((9, 5, 7), 2), ((6, 7, 5), 4), ((13, 9), 3),
+                                     ((5, 6, 4), 7), ((4, 5, 3), 12)])
+def test_ch_ilu_tensorial_mobility(cells, G):
+    """the Cahn-Hilliard block of the tensorial tangent: its c row's flux
+    coefficients found by evaluating the tangent at unit inputs"""
+    dim = len(cells)
+    h = (0.5, 0.75, 1.0)[:dim]
+    mesh = CartesianMesh(cells, (0.0,) * dim, tuple(n * hh for n, hh in zip(cells, h)))
+    nn = int(np.prod([n + 1 for n in cells]))
+    rng = np.random.default_rng(10 + G)
+    u = rng.uniform(0, 1, size=(nn, G + 2))
+    u[:, 1] = rng.uniform(-1, 1, nn)
+    check(mesh, G, u.ravel(), tens=True)
 
 
 def test_ch_ilu_on_the_packing():
