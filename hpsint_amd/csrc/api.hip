@@ -784,9 +784,6 @@ static int precond_update(hps_operator op, int kind, bool fp32)
     {
       if (op->q2)
         return fail(HPS_ERR_UNSUPPORTED, "the Cahn-Hilliard ILU preconditioner is built for Q1");
-      if (op->n_constrained)
-        return fail(HPS_ERR_UNSUPPORTED,
-                    "the Cahn-Hilliard ILU preconditioner takes no constrained nodes");
       // the Allen-Cahn block: the point-Jacobi inverse diagonal
       if (int r = precond_update(op, HPS_PRECON_POINT_JACOBI, fp32))
         return r;
@@ -799,6 +796,12 @@ static int precond_update(hps_operator op, int kind, bool fp32)
         }
       hps::ch_assemble(ch_params(op), op->cache64, op->ch64, op->stream);
       HPS_LAUNCHED();
+      if (op->n_constrained)
+        {
+          hps::ch_constrain(ch_params(op), op->constrained, op->n_constrained, op->ch64,
+                            op->stream);
+          HPS_LAUNCHED();
+        }
       hps::ch_factor(ch_params(op), op->ch64, op->ch32, op->stream);
       HPS_LAUNCHED();
       op->precond_kind = kind;

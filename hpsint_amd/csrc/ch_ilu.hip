@@ -357,6 +357,62 @@ namespace hps
       }
   }
 
+  // constrained nodes (OperatorBase::vmult, S/operator_base.h:294-353: the
+  // constrained src entries are zeroed and the constrained rows are
+  // identity rows): the Cahn-Hilliard block of the constrained operator has
+  // identity rows and columns at those nodes.  One thread per constrained
+  // node: its row becomes the identity, and the entry of each neighbour's
+  // row that couples to it becomes zero.  Two neighbouring constrained
+  // nodes may zero the same entries; every write is a zero or the node's
+  // own identity block, so the order does not matter.
+  template <int DIM>
+  __global__ void __launch_bounds__(256)
+    k_ch_constrain(const ChParams cp, const int64_t *__restrict__ nodes, int64_t n,
+                   double *__restrict__ A)
+  {
+    constexpr int S = Sten<DIM>::S;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+      return;
+    const int64_t node = nodes[t];
+    const int     x = (int)(node % cp.nn[0]);
+    const int     y = (int)((node / cp.nn[0]) % cp.nn[1]);
+    const int     z = DIM == 3 ? (int)(node / ((int64_t)cp.nn[0] * cp.nn[1])) : 0;
+    auto row = [&](int xx, int yy, int zz, int64_t &base, int64_t &stride) {
+      const int col = (xx & 1) | ((yy & 1) << 1) | ((zz & 1) << 2);
+      base = cp.off[col] +
+             4 * ((xx >> 1) + (int64_t)cp.mx[col] * ((yy >> 1) + (int64_t)cp.my[col] * (zz >> 1)));
+      stride = 4 * cp.cnt[col];
+    };
+    int64_t b, st;
+    row(x, y, z, b, st);
+    const int centre = Sten<DIM>::index(0, 0, 0);
+    for (int s = 0; s < S; ++s)
+      {
+        double *e = A + b + s * st;
+        e[0]      = s == centre ? 1.0 : 0.0;
+        e[1]      = 0.0;
+        e[2]      = 0.0;
+        e[3]      = s == centre ? 1.0 : 0.0;
+      }
+    for (int dz = DIM == 3 ? -1 : 0; dz <= (DIM == 3 ? 1 : 0); ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+          {
+            const int xx = x + dx, yy = y + dy, zz = z + dz;
+            if ((dx == 0 && dy == 0 && dz == 0) || xx < 0 || yy < 0 || zz < 0 ||
+                xx >= cp.nn[0] || yy >= cp.nn[1] || zz >= cp.nn[2])
+              continue;
+            int64_t nb, nst;
+            row(xx, yy, zz, nb, nst);
+            double *e = A + nb + Sten<DIM>::index(-dx, -dy, -dz) * nst;
+            e[0]      = 0.0;
+            e[1]      = 0.0;
+            e[2]      = 0.0;
+            e[3]      = 0.0;
+          }
+  }
+
   // a node of colour COL with in-colour index ic: its place in the colour's
   // grid (ix, iy, iz), its coordinates and its node number
   template <int DIM, int COL>
@@ -759,6 +815,17 @@ namespace hps
       ch_assemble_tens_dim<3>(cp, par, n, cache, A, s);
     else
       ch_assemble_tens_dim<2>(cp, par, n, cache, A, s);
+  }
+
+  void ch_constrain(const ChParams &cp, const int64_t *nodes, int64_t n, double *A,
+                    hipStream_t s)
+  {
+    if (n <= 0)
+      return;
+    if (cp.dim == 3)
+      k_ch_constrain<3><<<blocks_for(n), 256, 0, s>>>(cp, nodes, n, A);
+    else
+      k_ch_constrain<2><<<blocks_for(n), 256, 0, s>>>(cp, nodes, n, A);
   }
 
   void ch_assemble(const ChParams &cp, const double *cache, double *A, hipStream_t s)

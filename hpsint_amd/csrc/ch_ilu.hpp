@@ -48,6 +48,9 @@ namespace hps
   // Cahn-Hilliard block of the tangent (FP64) from the FP64
   // linearisation cache (scalar or tensorial mobility, 1-12 order parameters)
   void ch_assemble(const ChParams &cp, const double *cache, double *A, hipStream_t s);
+  // identity rows and columns at the constrained nodes (n node indices)
+  void ch_constrain(const ChParams &cp, const int64_t *nodes, int64_t n, double *A,
+                    hipStream_t s);
   // in-place ILU(0) (FP64) and its FP32 copy
   void ch_factor(const ChParams &cp, double *A, float *A32, hipStream_t s);
   // dst = (LU)^-1 src on (c, mu), inv_diag * src on the order parameters

@@ -162,8 +162,9 @@ int hps_vmult_f32(hps_operator op, float *dst, const float *src);
  * c and mu rows and columns) from the linearisation cache, factors it by
  * ILU(0) with the nodes in 2^dim parity colours (each colour a parallel
  * stage; c before mu within a node), and takes the inverse diagonal on the
- * order parameters (Q1, scalar or tensorial mobility, no constrained
- * nodes; the assembly and factors are FP64, the FP32 apply uses an FP32
+ * order parameters (Q1, scalar or tensorial mobility; constrained nodes
+ * get identity rows and columns in the block, as in the constrained
+ * operator; the assembly and factors are FP64, the FP32 apply uses an FP32
  * copy). */
 int hps_precond_update(hps_operator op, int kind);
 /* The same setup for the FP32 inner Krylov solve: the diagonal / blocks are

@@ -87,6 +87,47 @@ def test_ch_ilu_tensorial_mobility(cells, G):
     check(mesh, G, u.ravel(), tens=True)
 
 
+@pytest.mark.parametrize("cells,G,tens", [((9, 5, 7), 2, False), ((13, 9), 3, False),
+                                          ((6, 7, 5), 4, True)])
+def test_ch_ilu_with_constrained_nodes(cells, G, tens):
+    """constrained nodes (x = 0 face plus random nodes): the ILU(0) of the
+    constrained operator's Cahn-Hilliard block, whose constrained rows and
+    columns are identity rows and columns; dst = src on those nodes"""
+    import scipy.sparse as sps
+    dim = len(cells)
+    h = (0.5, 0.75, 1.0)[:dim]
+    dt = 1e-3
+    mesh = CartesianMesh(cells, (0.0,) * dim, tuple(n * hh for n, hh in zip(cells, h)))
+    nn = int(np.prod([n + 1 for n in cells]))
+    nc = G + 2
+    rng = np.random.default_rng(20 + G)
+    u = rng.uniform(0, 1, size=(nn, nc))
+    u[:, 1] = rng.uniform(-1, 1, nn)
+    u = u.ravel()
+    nodes = np.unique(np.concatenate([np.arange(0, nn, cells[0] + 1),
+                                      rng.choice(nn, size=15, replace=False)]))
+    op = SinteringOperatorGeneric(mesh, G, FreeEnergyParams(), MobilityParams(tensorial=tens))
+    op.set_time_weights(1.0 / dt, (-1.0 / dt,))
+    p = po.make_params(dim, 1, mesh.n_cells, mesh.cell_size, nc, tensorial=tens,
+                       w0=1.0 / dt, w_hist=(-1.0 / dt,))
+    A = ps.sparse_tangent(p, po.fill_qp(p, u), keep_zeros=True)
+    mask = np.zeros(nn * nc)
+    mask[(nodes[:, None] * nc + np.arange(nc)).ravel()] = 1.0
+    D, P = sps.diags(1.0 - mask), sps.diags(mask)
+    Ac = (D @ A @ D + P).tocsr()
+    ref_apply = ps.parity_colour_ilu(Ac, nc, [n + 1 for n in cells])
+    op.fill_quadrature_point_values(dev(u))
+    op.set_constrained_nodes(nodes)
+    P2 = Preconditioner(op, "BlockPreconditioner2")
+    P2.do_update()
+    x = np.random.default_rng(7).standard_normal(u.size)
+    y = op.initialize_dof_vector()
+    P2.vmult(y, dev(x))
+    got, ref = host(y), ref_apply(x)
+    assert rel_l2(got, ref) <= 1e-12, rel_l2(got, ref)
+    assert np.array_equal(got.reshape(-1, nc)[nodes], x.reshape(-1, nc)[nodes])
+
+
 def test_ch_ilu_on_the_packing():
     big = packing.packing_cfg1(seed=0)
     h = (big.upper[0] - big.lower[0]) / 128
