@@ -12,7 +12,10 @@
 // z level is arithmetic on the lane's own registers.  Per component and
 // layer that is 6 lane exchanges in the interpolation and 6 in the
 // integration for two Gauss points -- half of what one lane per
-// (cell, corner) needs -- and no ds_swizzle.
+// (cell, corner) needs -- and no ds_swizzle.  The FP32 tangent with up to
+// 4 order parameters reads the interpolation's partner values (x, y and
+// diagonal corner of the cell face) from the LDS node planes instead of
+// exchanging them.
 //
 // March.  The workgroup marches along z through `tz` cell layers with two
 // node planes of 9 x 9 nodes in LDS (planes k and k+1 are a cell layer's
@@ -189,6 +192,51 @@ namespace hps
     const T pAh = xshfl<2>(Ah);
     const T pDl = xshfl<2>(Dl);
     const T pDh = xshfl<2>(Dh);
+    const T AAl = Al + r * pAl;
+    const T AAh = Ah + r * pAh;
+    const T Ayl = Al - pAl;
+    const T Ayh = Ah - pAh;
+    const T DAl = Dl + r * pDl;
+    const T DAh = Dh + r * pDh;
+    vl          = AAl + r * AAh;
+    vh          = AAh + r * AAl;
+    gzl         = AAl - AAh;
+    gxl         = DAl + r * DAh;
+    gxh         = DAh + r * DAl;
+    gyl         = Ayl + r * Ayh;
+    gyh         = Ayh + r * Ayl;
+  }
+
+  // the four nodes of the lane's cell face in a node plane of the LDS ring:
+  // its own corner, the x partner, the y partner and the diagonal one
+  template <typename T>
+  struct ZQuad
+  {
+    const T *o, *x, *y, *d;
+  };
+
+  // zcol_interp with the partners' values read from the ring instead of
+  // exchanged between lanes: the y partner's x-level sums are formed here
+  // from its two nodes, with the operations the partner lane would use, so
+  // the results are the same bit for bit
+  template <typename T>
+  __device__ __forceinline__ void zcol_interp_q(const ZQuad<T> &lo, const ZQuad<T> &hi, int c,
+                                                T &vl, T &vh, T &gxl, T &gxh, T &gyl, T &gyh,
+                                                T &gzl)
+  {
+    const T r   = Q1Hat<T>::r;
+    const T ul  = lo.o[c], uh = hi.o[c];
+    const T pl  = lo.x[c], ph = hi.x[c];
+    const T ql  = lo.y[c], qh = hi.y[c];
+    const T dl  = lo.d[c], dh = hi.d[c];
+    const T Al  = ul + r * pl;
+    const T Ah  = uh + r * ph;
+    const T Dl  = ul - pl;
+    const T Dh  = uh - ph;
+    const T pAl = ql + r * dl;
+    const T pAh = qh + r * dh;
+    const T pDl = ql - dl;
+    const T pDh = qh - dh;
     const T AAl = Al + r * pAl;
     const T AAh = Ah + r * pAh;
     const T Ayl = Al - pAl;
@@ -488,6 +536,12 @@ namespace hps
     constexpr bool TWO = VM && G > 4;
     constexpr int  NGV = TWO ? 2 : NC; // gradients interpolated with the values
     constexpr bool PF  = VM && sizeof(T) == 8 && !TWO;
+    // FP32 tangent with up to 4 order parameters: the interpolation's
+    // partner values from the ring (zcol_interp_q; 2-3 % faster on the
+    // 128^3 workload with 4; FP64 gains nothing and its allocation spills,
+    // FP32 with 8 or 12 loses 0.3-1.7 %)
+    constexpr bool LDSQ = VM && sizeof(T) == 4 && G <= 4;
+    const int      q_dx = (bx ? -1 : 1) * NC, q_dy = (by ? -PX : PX) * NC;
     T    lin_next[VM ? 2 : 1][VM ? NF : 1];
     auto load_lin = [&](int k) {
       if constexpr (VM)
@@ -530,6 +584,8 @@ namespace hps
           load_lin(k);
         const T *plo = &ring[k & 1][0][pn_lane * NC];
         const T *phi = &ring[(k + 1) & 1][0][pn_lane * NC];
+        const ZQuad<T> qlo{plo, plo + q_dx, plo + q_dy, plo + q_dx + q_dy};
+        const ZQuad<T> qhi{phi, phi + q_dx, phi + q_dy, phi + q_dx + q_dy};
         if constexpr (!VM)
           {
             T val[2][NC], gx[2][NC], gy[2][NC], gz[NC], hist[2][NC];
@@ -592,8 +648,14 @@ namespace hps
 #pragma unroll
             for (int c = 0; c < NC; ++c)
               if (c < NGV)
-                zcol_interp(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c], gy[0][c],
-                            gy[1][c], gz[c]);
+                {
+                  if constexpr (LDSQ)
+                    zcol_interp_q(qlo, qhi, c, val[0][c], val[1][c], gx[0][c], gx[1][c],
+                                  gy[0][c], gy[1][c], gz[c]);
+                  else
+                    zcol_interp(plo[c], phi[c], val[0][c], val[1][c], gx[0][c], gx[1][c],
+                                gy[0][c], gy[1][c], gz[c]);
+                }
               else
                 zcol_interp_values(plo[c], phi[c], val[0][c], val[1][c]);
             __builtin_amdgcn_sched_barrier(0);
