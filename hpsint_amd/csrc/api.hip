@@ -69,7 +69,10 @@ struct hps_operator_s
   int                  n_tiles, tz;
   // 3D scalar tangent with z-column lanes (8 x 8 x tz tiles, ntz1 along y)
   bool                 zcol, zcol32, zcol_residual;
-  int                  ntz1, n_tiles_z;
+  int                  ntz1;
+  // layers per z-column march of the FP64 tangent and the residual (a
+  // multiple of the cache tile depth tz; the FP32 tangent marches tz)
+  int                  zcol_mz;
   int64_t              n_nodes, n_dofs, cache_elems, part_elems;
   hipStream_t          stream;
   double              *cache64;
@@ -119,6 +122,26 @@ static hps::Q2Params q2_params(hps_operator op)
   for (int i = 0; i < 3; ++i)
     q.w_hist[i] = op->kp.w_hist[i];
   return q;
+}
+
+// launch parameters of the z-column kernels: marches of zcol_mz layers
+// (wide) or of one cache tile, over the cache tiles of kp.tz layers
+static hps::KParams zcol_params(hps_operator op, bool wide)
+{
+  hps::KParams k = op->kp;
+  k.ctz          = k.tz;
+  if (wide)
+    {
+      k.tz    = op->zcol_mz;
+      k.nt[2] = (k.nc[2] + k.tz - 1) / k.tz;
+    }
+  return k;
+}
+
+// workgroups of a z-column launch with parameters zk
+static int zcol_tiles(hps_operator op, const hps::KParams &zk)
+{
+  return zk.nt[0] * op->ntz1 * zk.nt[2];
 }
 
 // event i (0: before the main kernel, 1: before the fix-up, 2: after it) of
@@ -301,11 +324,21 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
       op->zcol32 = zc && G <= hps::ZCOL32_MAX_G;
       // the residual with z-column lanes as well (HPSINT_AMD_ZCOL_RES=0: not)
       op->zcol_residual = op->zcol && env_int("HPSINT_AMD_ZCOL_RES", 1) != 0;
+      // FP64 tangent and residual: marches of HPSINT_AMD_ZCOL_MARCH cache
+      // tiles (default 2: 32 layers, half the tile faces along z; with 2
+      // workgroups per CU the 128^3 grid still fills the chip), FP32: one
+      // (3 workgroups per CU; two tiles per march left a tail)
+      const int zm = env_int("HPSINT_AMD_ZCOL_MARCH", 2);
+      if (zm < 1 || zm > 64)
+        return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_ZCOL_MARCH out of range");
+      op->zcol_mz = zm * TZ;
       if (op->zcol || op->zcol32)
         {
-          op->ntz1       = (ncy + 7) / 8;
-          op->n_tiles_z  = kp.nt[0] * op->ntz1 * kp.nt[2];
-          op->part_elems = std::max(op->part_elems, hps::zcol_part_elems(kp, op->ntz1, op->nc));
+          op->ntz1 = (ncy + 7) / 8;
+          for (const bool wide : {false, true})
+            op->part_elems =
+              std::max(op->part_elems, hps::zcol_part_elems(zcol_params(op, wide), op->ntz1,
+                                                            op->nc));
         }
       if ((e = hipMalloc(&op->part, op->part_elems * sizeof(double))) != hipSuccess)
         return cleanup_fail(e);
@@ -535,16 +568,18 @@ extern "C" int hps_set_linearization(hps_operator op, const double *u)
 }
 
 // sums the tile-face partials of the Q1 kernels (the Q2 colour passes
-// complete dst themselves)
+// complete dst themselves); zcol_tiles: of the z-column kernel, whose
+// marches are zcol_mz layers deep if zcol_wide
 template <typename T>
 static int finish_fixup(hps_operator op, T *dst, int ostride, int ocstride, int ooffset,
-                        bool zcol_tiles = false)
+                        bool zcol_tiles = false, bool zcol_wide = false)
 {
   if (op->q2)
     return HPS_OK;
   if (zcol_tiles)
     {
-      hps::fixup_zcol_d3(op->kp, op->ntz1, op->nc, (const T *)op->part, dst, op->stream);
+      hps::fixup_zcol_d3(zcol_params(op, zcol_wide), op->ntz1, op->nc, (const T *)op->part, dst,
+                         op->stream);
       HPS_LAUNCHED();
       return HPS_OK;
     }
@@ -584,15 +619,18 @@ extern "C" int hps_residual(hps_operator op, int wtd, double *dst, const double 
   if (op->q2)
     op->q2->residual(q2_params(op), w, src, o[0], o[1], o[2], dst, op->stream);
   else if (zres)
-    hps::residual_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, w, src, o[0], o[1],
-                          o[2], dst, op->part, op->stream);
+    {
+      const hps::KParams zk = zcol_params(op, true);
+      hps::residual_zcol_d3(op->n_grains, zk, op->ntz1, zcol_tiles(op, zk), w, src, o[0], o[1],
+                            o[2], dst, op->part, op->stream);
+    }
   else
     (op->cut_active ? op->ops->residual_cut : op->ops->residual)(
       op->kp, op->n_tiles, w, src, o[0], o[1], o[2], dst, op->part, op->stream);
   HPS_LAUNCHED();
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
-  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0, zres))
+  if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0, zres, true))
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));
@@ -667,8 +705,11 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
                             op->cut_masks, src, op->cache64, dst, op->part, op->stream);
       }
   else if (op->zcol && !op->cut_active)
-    hps::vmult_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, src, op->cache64, dst,
-                       op->part, op->stream);
+    {
+      const hps::KParams zk = zcol_params(op, true);
+      hps::vmult_zcol_d3(op->n_grains, zk, op->ntz1, zcol_tiles(op, zk), src, op->cache64, dst,
+                         op->part, op->stream);
+    }
   else
     (op->cut_active ? op->ops->vmult64_cut : op->ops->vmult64)(op->kp, op->n_tiles, src, dst,
                                                                op->part, op->cache64,
@@ -677,7 +718,7 @@ extern "C" int hps_vmult_f64(hps_operator op, double *dst, const double *src)
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 1), op->stream));
   if (int r = finish_fixup<double>(op, dst, op->nc, 1, 0,
-                                   !op->cut_ready && op->zcol && !op->cut_active))
+                                   !op->cut_ready && op->zcol && !op->cut_active, true))
     return r;
   if (op->timing)
     HPS_HIP(hipEventRecord(timing_event(op, 2), op->stream));
@@ -719,8 +760,11 @@ extern "C" int hps_vmult_f32(hps_operator op, float *dst, const float *src)
                             op->stream);
       }
   else if (op->zcol32 && !op->cut_active)
-    hps::vmult_zcol_d3(op->n_grains, op->kp, op->ntz1, op->n_tiles_z, src, op->cache32, dst,
-                       (float *)op->part, op->stream);
+    {
+      const hps::KParams zk = zcol_params(op, false);
+      hps::vmult_zcol_d3(op->n_grains, zk, op->ntz1, zcol_tiles(op, zk), src, op->cache32, dst,
+                         (float *)op->part, op->stream);
+    }
   else
     (op->cut_active ? op->ops->vmult32_cut : op->ops->vmult32)(
       op->kp, op->n_tiles, src, dst, (float *)op->part, op->cache32, op->stream);

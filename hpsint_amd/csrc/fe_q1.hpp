@@ -46,6 +46,8 @@ namespace hps
     int    nn[3];     // nodes along x, y (1 in 2D), march direction
     int    nt[3];     // tiles along x, y, march
     int    tz;        // cells per tile along the march direction
+    int    ctz;       // cells per cache tile along the march (0: tz); the
+                      // z-column kernels march tz = a multiple of ctz
     int    order;     // number of history vectors (BDF order)
     double inv_h[3];  // 1/h along the physical directions x, y, z
     double jxw;       // prod(h) * (1/2)^DIM

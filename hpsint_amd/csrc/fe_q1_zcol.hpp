@@ -402,8 +402,19 @@ namespace hps
     int tyc = 2 * tiy + (cy >> 2);
     if (tyc >= kp.nt[1])
       tyc = 2 * tiy;
-    const int64_t ctile   = tix + (int64_t)kp.nt[0] * (tyc + (int64_t)kp.nt[1] * tim);
-    const T      *cache_t = cache + ctile * kp.tz * NF * NTC;
+    // cache tiles are ctz layers deep; the march of tz layers crosses
+    // tz / ctz of them
+    // (FP64, MW: the lane's x/y tile in a pointer; the cache tile along z
+    // and the layer in it are counters that load_lin advances by one layer
+    // per call, the layers being requested in order from the march's
+    // first.  FP32 marches one cache tile: kp.ctz = kp.tz, the tile's
+    // layers at a fixed stride from cache_t; the counters cost it 3 %)
+    constexpr bool MW      = sizeof(T) == 8;
+    const int      ctz     = MW && kp.ctz > 0 ? kp.ctz : kp.tz;
+    const int64_t  ctile   = tix + (int64_t)kp.nt[0] * (tyc + (MW ? 0 : (int64_t)kp.nt[1] * tim));
+    const T       *cache_t = cache + ctile * ctz * NF * NTC;
+    const int64_t  cslab   = (int64_t)kp.nt[0] * kp.nt[1] * ctz;
+    int            lin_ct = MW ? tim * (kp.tz / ctz) : 0, lin_layer = 0;
     const int     lane_lo = ((((cy & 3) * 8) + cx) << 3) | xy; // corner with z bit 0
     const int     lane_hi = lane_lo | 4;
 
@@ -543,10 +554,21 @@ namespace hps
     constexpr bool LDSQ = VM && sizeof(T) == 4 && G <= 4;
     const int      q_dx = (bx ? -1 : 1) * NC, q_dy = (by ? -PX : PX) * NC;
     T    lin_next[VM ? 2 : 1][VM ? NF : 1];
-    auto load_lin = [&](int k) {
+    auto load_lin = [&](int k) { // (called for k = 0, 1, ... in order)
       if constexpr (VM)
         {
-          const T *cl = cache_t + (int64_t)k * (NF * NTC);
+          const T *cl;
+          if constexpr (MW)
+            {
+              cl = cache_t + (cslab * lin_ct + lin_layer) * (NF * NTC);
+              if (++lin_layer == ctz)
+                {
+                  lin_layer = 0;
+                  ++lin_ct;
+                }
+            }
+          else
+            cl = cache_t + (int64_t)k * (NF * NTC);
 #pragma unroll
           for (int f = 0; f < NF; ++f)
             {

@@ -133,37 +133,46 @@ def test_every_node_written_exactly_once(dim, ncells, tz):
         assert all(c == 1 for c in counts), [i for i, c in enumerate(counts) if c != 1][:10]
 
 
+@pytest.mark.parametrize("march", [1, 2, 3])
 @pytest.mark.parametrize("tz", [16, 5])
 @pytest.mark.parametrize("ncells", [c for d, c in CASES if d == 3])
-def test_zcol_cache_map(ncells, tz):
+def test_zcol_cache_map(ncells, tz, march):
     """q1_vmult_zcol (hpsint_amd/csrc/fe_q1_zcol.hpp) reads the cache of the
-    8 x 4 tiles: for every lane of every 8 x 8 tile, the cache tile and the
-    lanes of its two Gauss points are those the 8 x 4 tiling gives the same
-    cell and corners; lanes of cells beyond the mesh read inside the cache."""
+    8 x 4 x tz tiles: for every lane of every 8 x 8 x (march tz) tile and
+    every layer of its march, the cache layer and the lanes of its two Gauss
+    points are those the 8 x 4 x tz tiling gives the same cell and corners;
+    lanes of cells beyond the mesh read inside the cache."""
     nc, nn, ty, nt, PX, PY = tiling(3, ncells, 8, 4, tz)
     ntz1 = -(-nc[1] // 8)
     n_tiles = nt[0] * nt[1] * nt[2]
-    for b in range(nt[0] * ntz1 * nt[2]):
+    mz = march * tz  # layers per march (kp.tz of the launch; kp.ctz = tz)
+    ntm = -(-nc[2] // mz)
+    for b in range(nt[0] * ntz1 * ntm):
         tix, r = b % nt[0], b // nt[0]
         tiy, tim = r % ntz1, r // ntz1
-        x0, y0 = tix * 8, tiy * 8
+        x0, y0, m0 = tix * 8, tiy * 8, tim * mz
         tx_eff, ty_eff = min(8, nc[0] - x0), min(8, nc[1] - y0)
+        tm_eff = min(mz, nc[2] - m0)
         for tid in range(256):
             xy, cell = tid & 3, tid >> 2
             cx, cy = cell & 7, cell >> 3
             tyc = 2 * tiy + (cy >> 2)
             if tyc >= nt[1]:
                 tyc = 2 * tiy
-            ctile = tix + nt[0] * (tyc + nt[1] * tim)
-            assert 0 <= ctile < n_tiles
+            cxy = tix + nt[0] * tyc
             lane_lo = (((cy & 3) * 8 + cx) << 3) | xy
             assert 0 <= lane_lo and (lane_lo | 4) < 256
-            if cx < tx_eff and cy < ty_eff:
-                gy = y0 + cy
-                ref_tile = tix + nt[0] * (gy // 4 + nt[1] * tim)
-                ref_lane = ((gy % 4) * 8 + cx) * 8
-                assert ctile == ref_tile
-                assert lane_lo == ref_lane + xy and (lane_lo | 4) == ref_lane + xy + 4
+            for k in range(tm_eff):
+                gz = m0 + k
+                ct = gz // tz
+                layer = (cxy + nt[0] * nt[1] * ct) * tz + (gz - ct * tz)
+                assert 0 <= layer < n_tiles * tz
+                if cx < tx_eff and cy < ty_eff:
+                    gy = y0 + cy
+                    ref_tile = tix + nt[0] * (gy // 4 + nt[1] * (gz // tz))
+                    assert layer == ref_tile * tz + gz % tz
+                    ref_lane = ((gy % 4) * 8 + cx) * 8
+                    assert lane_lo == ref_lane + xy and (lane_lo | 4) == ref_lane + xy + 4
 
 
 def zcol_slot_layout(ncells, tz):
@@ -191,7 +200,7 @@ def zcol_slot_layout(ncells, tz):
     return nc, nn, tsz, nt, nb, (n_m, n_y, n_x), record
 
 
-@pytest.mark.parametrize("tz", [16, 5, 3])
+@pytest.mark.parametrize("tz", [32, 16, 5, 3])
 @pytest.mark.parametrize("ncells", [c for d, c in CASES if d == 3] + [(17, 9, 33), (8, 16, 4)])
 def test_zcol_slot_partials(ncells, tz):
     """put() of q1_vmult_zcol and q1_fixup_slots: every node written once,
