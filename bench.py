@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # scaled by the same counter's reading for a copy of known size; counts and
 # factors in profiles/vmult_q1_128cubed_hbm_traffic.md (z-column kernel).
 # Valid for that workload only.
-VMULT_TRAFFIC_128 = 1.3687e9
+VMULT_TRAFFIC_128 = 1.3612e9
 # FP64 flop per launch of the residual main kernel (z-column lanes) on the
 # same workload (SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 x 64 lanes, FMA = 2;
 # profiles/q1_kernels_128cubed_fp64_counters.md); FP64 vector peak
