@@ -191,6 +191,12 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   const int TZ     = env_int("HPSINT_AMD_TILE_M", 16);
   if (TZ < 1 || TZ > 4096)
     return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_TILE_M out of range");
+  // Q1 3D z-column FP64 tangent and residual: most cache tiles per march
+  // and the workgroups per CU a deeper march must leave
+  const int zm    = env_int("HPSINT_AMD_ZCOL_MARCH", 2);
+  const int zm_wg = env_int("HPSINT_AMD_ZCOL_MARCH_WG", 4);
+  if (zm < 1 || zm > 64 || zm_wg < 0 || zm_wg > 1 << 20)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_ZCOL_MARCH(_WG) out of range");
   // Q2 3D tangent vmult: cell layers per tile march (0: colour passes)
   const int q2_tile = env_int("HPSINT_AMD_Q2_TILE", 2);
   if (q2_tile < 0 || q2_tile > 4096)
@@ -324,17 +330,31 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
       op->zcol32 = zc && G <= hps::ZCOL32_MAX_G;
       // the residual with z-column lanes as well (HPSINT_AMD_ZCOL_RES=0: not)
       op->zcol_residual = op->zcol && env_int("HPSINT_AMD_ZCOL_RES", 1) != 0;
-      // FP64 tangent and residual: marches of HPSINT_AMD_ZCOL_MARCH cache
-      // tiles (default 2: 32 layers, half the tile faces along z; with 2
-      // workgroups per CU the 128^3 grid still fills the chip), FP32: one
-      // (3 workgroups per CU; two tiles per march left a tail)
-      const int zm = env_int("HPSINT_AMD_ZCOL_MARCH", 2);
-      if (zm < 1 || zm > 64)
-        return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_ZCOL_MARCH out of range");
-      op->zcol_mz = zm * TZ;
+      // FP64 tangent and residual: marches of up to HPSINT_AMD_ZCOL_MARCH
+      // cache tiles (default 2: 32 layers, half the tile faces along z),
+      // the deepest that still gives HPSINT_AMD_ZCOL_MARCH_WG (default 4)
+      // workgroups per CU (2 run at once at 2 waves/SIMD; 128^3: two tiles,
+      // 64^3: one); FP32: one (3 workgroups per CU; two tiles per march
+      // left a tail)
+      int dev = 0, cus = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          cus < 1)
+        {
+          (void)hipGetLastError();
+          cus = 256; // MI355X
+        }
+      op->ntz1    = (ncy + 7) / 8;
+      op->zcol_mz = TZ;
+      for (int m = zm; m > 1; --m)
+        if ((int64_t)kp.nt[0] * op->ntz1 * ((ncm + m * TZ - 1) / (m * TZ)) >=
+            (int64_t)zm_wg * cus)
+          {
+            op->zcol_mz = m * TZ;
+            break;
+          }
       if (op->zcol || op->zcol32)
         {
-          op->ntz1 = (ncy + 7) / 8;
           for (const bool wide : {false, true})
             op->part_elems =
               std::max(op->part_elems, hps::zcol_part_elems(zcol_params(op, wide), op->ntz1,

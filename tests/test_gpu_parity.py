@@ -501,6 +501,29 @@ def test_vmult_zcol_and_corner_lanes(monkeypatch, zcol, cells, G):
     assert max(errs) <= 1e-5, errs
 
 
+@pytest.mark.parametrize("march", ["1", "2", "3"])
+def test_zcol_march_depths(monkeypatch, march):
+    """FP64 tangent and residual with z-column marches of one and of three
+    cache tiles (16 layers each): 37 cells along z end inside a march and
+    inside a cache tile; the oracle's vmult and residual at 1e-12 (no floor
+    on the workgroups per CU, so the small grid marches that deep)"""
+    monkeypatch.setenv("HPSINT_AMD_ZCOL_MARCH", march)
+    monkeypatch.setenv("HPSINT_AMD_ZCOL_MARCH_WG", "0")
+    cells, G = (19, 13, 37), 4
+    op, p = make_pair(3, cells, G, False, w0=100.0, w_hist=(-100.0,))
+    u = fields(p, 3)
+    src = np.random.default_rng(4).uniform(-1, 1, size=u.size)
+    ref = po.vmult(p, po.fill_qp(p, u), src)
+    op.fill_quadrature_point_values(dev(u))
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, dev(src))
+    assert rel_l2(host(dst), ref) <= TOL64, rel_l2(host(dst), ref)
+    olds = [fields(p, 6)]
+    ref = po.residual(p, 2, u, olds)
+    op.evaluate_nonlinear_residual(dst, dev(u), 2, old=[dev(o) for o in olds])
+    assert rel_l2(host(dst), ref) <= TOL64, rel_l2(host(dst), ref)
+
+
 @pytest.mark.parametrize("zcol", ["0", "1"])
 @pytest.mark.parametrize("cells,G", [((19, 13, 37), 4), ((9, 5, 17), 2), ((3, 7, 2), 3),
                                      ((8, 12, 16), 1)])
