@@ -485,7 +485,8 @@ namespace hps
   // grad eta_i) and is formed from the Gauss-point values before the
   // integration.
   // cl: this lane's entry of the first cached field (fields LP entries apart)
-  template <int DIM, int G, bool TENS, typename T, bool WAVE = false>
+  // EL: request all cache fields at the top (the tile march, FP64)
+  template <int DIM, int G, bool TENS, typename T, bool WAVE = false, bool EL = false>
   __device__ __forceinline__ void q2_vmult_body(const Q2Params &kp, const Q2Lane<DIM> &L, T *U,
                                                 T *VR, T *A, const T *__restrict__ cl,
                                                 const T *inv_h)
@@ -498,13 +499,25 @@ namespace hps
 
     // the tensorial Q_i fields are requested before the interpolation
     // passes, so that they arrive while the passes run
+    // EL: the other cache fields as well (their latency is otherwise
+    // waited for after the interpolation passes: the FP64 tile march 4 %
+    // faster at 96^3 with 10 order parameters; FP32 1.4 % slower, and the
+    // colour-pass kernels lose occupancy)
     constexpr bool EARLYQ = TENS && G > 1;
+    constexpr bool EARLYL = EARLYQ && EL;
     T              qf[EARLYQ ? G * DIM : 1];
+    T              lin_e[EARLYL ? CL::gb_q : 1];
     if constexpr (EARLYQ)
       {
 #pragma unroll
         for (int f = 0; f < G * DIM; ++f)
           qf[f] = L.valid ? cl[(CacheLayout<DIM, G, TENS>::gb_q + f) * Q2Lane<DIM>::LP] : T(0);
+      }
+    if constexpr (EARLYL)
+      {
+#pragma unroll
+        for (int f = 0; f < CL::gb_q; ++f)
+          lin_e[f] = L.valid ? cl[f * Q2Lane<DIM>::LP] : T(0);
       }
     // 1) values at the Gauss points, one pass per direction for all
     //    components (U -> A -> U -> registers in 3D, U -> A -> registers in 2D)
@@ -578,7 +591,10 @@ namespace hps
           T lin[CL::gb_q];
 #pragma unroll
           for (int f = 0; f < CL::gb_q; ++f)
-            lin[f] = L.valid ? cl[f * LP] : T(0);
+            if constexpr (EARLYL)
+              lin[f] = lin_e[f];
+            else
+              lin[f] = L.valid ? cl[f * LP] : T(0);
           tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
           tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0, qsum);
         }
@@ -1005,7 +1021,7 @@ namespace hps
         const int64_t ci  = ((int64_t)(cz >> 1) * ncy + (cy >> 1)) * ncx + (cx >> 1);
         const T      *cl  = cache + ((tp.wgb[col] + (ci >> 1)) * NF) * LN::LP +
                       (int)(ci & 1) * LN::NP + L.p;
-        q2_vmult_body<3, G, TENS, T, true>(kq, L, U[w], H[w], A[w], cl, inv_hk);
+        q2_vmult_body<3, G, TENS, T, true, sizeof(T) == 8>(kq, L, U[w], H[w], A[w], cl, inv_hk);
         __syncthreads();
         // the next layer's src rows, in flight during the node sums (set on
         // both paths, so that this layer's rows are dead after the gather;
