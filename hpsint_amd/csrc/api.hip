@@ -195,8 +195,10 @@ extern "C" int hps_create(const hps_params *prm, hps_operator *out)
   // and the workgroups per CU a deeper march must leave
   const int zm    = env_int("HPSINT_AMD_ZCOL_MARCH", 2);
   const int zm_wg = env_int("HPSINT_AMD_ZCOL_MARCH_WG", 4);
-  if (zm < 1 || zm > 64 || zm_wg < 0 || zm_wg > 1 << 20)
-    return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_ZCOL_MARCH(_WG) out of range");
+  if (zm < 1 || zm > 64)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_ZCOL_MARCH out of range (1-64)");
+  if (zm_wg < 0 || zm_wg > 1 << 20)
+    return fail(HPS_ERR_INVALID_ARGUMENT, "HPSINT_AMD_ZCOL_MARCH_WG out of range");
   // Q2 3D tangent vmult: cell layers per tile march (0: colour passes)
   const int q2_tile = env_int("HPSINT_AMD_Q2_TILE", 2);
   if (q2_tile < 0 || q2_tile > 4096)

@@ -51,3 +51,23 @@ def test_adapter_library_loads_in_process():
     for name in ("hpsad_create", "hpsad_destroy", "hpsad_set_time", "hpsad_fill",
                  "hpsad_vmult", "hpsad_residual", "hpsad_precond", "hpsad_hook_calls"):
         assert hasattr(A, name), name
+
+
+@pytest.mark.parametrize("name,value", [("HPSINT_AMD_ZCOL_MARCH", "0"),
+                                        ("HPSINT_AMD_ZCOL_MARCH", "65"),
+                                        ("HPSINT_AMD_ZCOL_MARCH_WG", "-1"),
+                                        ("HPSINT_AMD_TILE_M", "0"),
+                                        ("HPSINT_AMD_Q2_TILE", "-1")])
+def test_tuning_switches_out_of_range_fail_without_gpu(monkeypatch, name, value):
+    """hps_create checks the tuning switches of its environment before it
+    touches the device: a value out of range is an invalid argument"""
+    monkeypatch.setenv(name, value)
+    L = _ffi.lib()
+    h = ctypes.c_void_p()
+    p = _ffi.HpsParams()
+    p.dim, p.fe_degree, p.n_components = 3, 1, 6
+    for d in range(3):
+        p.n_cells[d] = 2
+        p.cell_size[d] = 1.0
+    assert L.hps_create(ctypes.byref(p), ctypes.byref(h)) == _ffi.HPS_ERR_INVALID_ARGUMENT
+    assert name.encode() in L.hps_last_error()
