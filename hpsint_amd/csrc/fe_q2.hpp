@@ -983,7 +983,10 @@ namespace hps
     // the rows of layer k + 1 are requested after the body of layer k
     T    xs[NC];
     auto load_src = [&](int k) {
-      int lq = l; // (recomputed per call, as the lane values below)
+      // the lane index through an empty asm statement, as in the march
+      // below: the point index and address are formed per call, not held
+      // in registers across the cell body
+      int lq = l;
       asm volatile("" : "+v"(lq));
       const int cc[3] = {cx, cy, z0 + k};
       const int p     = lq - (lq / LN::NP) * LN::NP;
