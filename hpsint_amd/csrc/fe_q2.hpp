@@ -499,14 +499,16 @@ namespace hps
 
     // the tensorial Q_i fields are requested before the interpolation
     // passes, so that they arrive while the passes run
-    // EL: the other cache fields as well (their latency is otherwise
-    // waited for after the interpolation passes: the FP64 tile march 4 %
-    // faster at 96^3 with 10 order parameters; FP32 1.4 % slower, and the
-    // colour-pass kernels lose occupancy)
+    // EL: the other cache fields as well, or all of them with scalar
+    // mobility (their latency is otherwise waited for after the
+    // interpolation passes: the FP64 tile march at 96^3 with 10 order
+    // parameters 4 % faster tensorial, 3 % scalar; FP32 1.4 % slower, and
+    // the colour-pass kernels lose occupancy)
     constexpr bool EARLYQ = TENS && G > 1;
-    constexpr bool EARLYL = EARLYQ && EL;
+    constexpr bool EARLYL = EL;
+    constexpr int  NL     = EARLYQ ? CL::gb_q : NF; // the other fields
     T              qf[EARLYQ ? G * DIM : 1];
-    T              lin_e[EARLYL ? CL::gb_q : 1];
+    T              lin_e[EARLYL ? NL : 1];
     if constexpr (EARLYQ)
       {
 #pragma unroll
@@ -516,7 +518,7 @@ namespace hps
     if constexpr (EARLYL)
       {
 #pragma unroll
-        for (int f = 0; f < CL::gb_q; ++f)
+        for (int f = 0; f < NL; ++f)
           lin_e[f] = L.valid ? cl[f * Q2Lane<DIM>::LP] : T(0);
       }
     // 1) values at the Gauss points, one pass per direction for all
@@ -603,7 +605,10 @@ namespace hps
           T lin[NF];
 #pragma unroll
           for (int f = 0; f < NF; ++f)
-            lin[f] = L.valid ? cl[f * LP] : T(0);
+            if constexpr (EARLYL)
+              lin[f] = lin_e[f];
+            else
+              lin[f] = L.valid ? cl[f * LP] : T(0);
           tangent_values<DIM, G, TENS>(kp.pp, lin, v, vr);
           tangent_grad_c<DIM, G, TENS>(kp.pp, lin, v, g0, g1, gc0);
         }
