@@ -609,6 +609,16 @@ namespace hps
           load_plane(k + 2);
 
         // ---- cell layer k: interpolate, physics, integrate
+        // TWO without the cut-off: the layer's cache fields are requested
+        // before the interpolation (the barrier below keeps the physics
+        // after it; the cut-off kernel with 8 order parameters would lose
+        // its third wave per SIMD)
+        constexpr bool TOP = TWO && !CUT;
+        T              lin_top[TOP ? NF : 1];
+        if constexpr (TOP)
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            lin_top[f] = cache_tile[k * (NF * NT) + f * NT + tid];
         const T *pl = &ring[(k + t.bm) & 1][0][t.pn_lane * NC];
         T        val[NC], grad[NC][DIM];
 #pragma unroll
@@ -651,6 +661,10 @@ namespace hps
 #pragma unroll
               for (int f = 0; f < NF; ++f)
                 lin[f] = lin_next[f];
+            else if constexpr (TOP)
+#pragma unroll
+              for (int f = 0; f < NF; ++f)
+                lin[f] = lin_top[f];
             else
 #pragma unroll
               for (int f = 0; f < NF; ++f)
